@@ -1,0 +1,127 @@
+"""Benchmark: the springy step of a 1024 x 1024 (1,048,576-vertex) grid cloth draping over a
+sphere collider on one MI355X.
+
+One "step" is one Simulation::step of the whole cloth (forces, RK4, collision, pins) at
+dt = 1/600 s.  The cloth's state is resident in HBM before the timed region; the timed region is
+exactly K steps bracketed by device synchronisations.
+
+Prints ONE JSON line:
+  value           simulated seconds per wall second (K * dt / wall time)
+  roofline        the step kernel against HBM bandwidth: 48 algorithmic bytes per vertex-step
+                  (x, y, z, vx, vy, vz read once and written once, f32) over the kernel's average
+                  duration measured with HIP events on the simulation's stream
+  cpu_baseline    the oracle (a single-threaded C++ restatement of the reference step) on a bounded
+                  sample of the same workload, on this host's cores
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_VERTEX_STEP = 48
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=600)
+    ap.add_argument("--warmup", type=int, default=60)
+    ap.add_argument("--n", type=int, default=1024, help="cloth side (vertices)")
+    ap.add_argument("--math", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--kernel", choices=["auto", "grid", "csr"], default="auto")
+    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import feriphys_amd as fa
+    from feriphys_amd import workloads as wl
+
+    scene = wl.config1(n=args.n, steps=args.steps)
+    sim = wl.build_gpu(scene)
+    kern = {"auto": fa.FP_KERNEL_AUTO, "grid": fa.FP_KERNEL_GRID, "csr": fa.FP_KERNEL_CSR}[args.kernel]
+    sim.set_kernel(kern, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
+    N = scene.cloths[0].rows * scene.cloths[0].cols
+    dt = scene.config.dt
+
+    # Warm up on the same scene, then restore its initial state so that the timed region is
+    # exactly steps 1..K of the workload (the restore is outside the timed region).
+    initial = sim.get_points(0)
+    if args.warmup:
+        sim.step(args.warmup)
+        sim.set_points(0, initial)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms = sim.time_steps(args.steps)  # HIP events on the simulation's stream around the K steps
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+
+    pts = sim.get_points(0)
+    if not np.isfinite(pts).all():
+        print("non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(3)
+
+    sim_s_per_s = args.steps * dt / wall
+    ms_per_step = wall * 1e3 / args.steps
+    kernel_avg_s = kernel_ms * 1e-3 / args.steps
+    achieved = BYTES_PER_VERTEX_STEP * N / kernel_avg_s / 1e9
+
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        o = orc.build_scene(wl.config1(n=args.n, steps=args.cpu_steps))
+        o.step(1)  # warm-up
+        t = time.perf_counter()
+        o.step(args.cpu_steps)
+        cpu_wall = time.perf_counter() - t
+        cpu = {
+            "value": args.cpu_steps * dt / cpu_wall,
+            "unit": "sim-s/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{args.cpu_steps} steps of the same {args.n}x{args.n} cloth + sphere after 1 warm-up step "
+                      f"(single-threaded C++ restatement of the reference step, -O2, no FMA); host "
+                      f"{platform.processor() or platform.machine()}, nproc {os.cpu_count()}",
+            "vertex_steps_per_s": args.cpu_steps * N / cpu_wall,
+        }
+
+    line = {
+        "metric": "sim-s/s, 1M-vert cloth: surrogate vs plain @<=1e-3 pos err; %HBM peak",
+        "value": sim_s_per_s,
+        "unit": "sim-s/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain springy step, {args.n}x{args.n} grid cloth ({N} vertices), top edge pinned, "
+                               f"320-face icosphere collider, RK4 dt=1/600 s",
+                   "vertices": N, "dt": dt, "kernel": args.kernel, "math": args.math},
+        "vertex_steps_per_s": args.steps * N / wall,
+        "surrogate": None,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": BYTES_PER_VERTEX_STEP * N},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

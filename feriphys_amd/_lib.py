@@ -1,0 +1,127 @@
+"""Loader for libferiphys_gpu.so (the gfx950 springy step behind the C ABI in include/feriphys_gpu.h).
+
+There is no CPU fallback: if the library is missing or fails to load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libferiphys_gpu.so")
+
+FP_OK, FP_EINVAL, FP_ENONMANIFOLD, FP_EHIP, FP_ESTATE, FP_ENOMEM, FP_ENODEV = range(7)
+FP_EULER, FP_RK4 = 0, 1
+FP_PLANE_XY, FP_PLANE_XZ = 0, 1
+FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID = 0, 1, 2
+FP_MATH_EXACT, FP_MATH_FAST = 0, 1
+
+# Every symbol declared in include/feriphys_gpu.h, with its ctypes signature.
+_U64 = C.c_uint64
+_U64P = C.POINTER(C.c_uint64)
+_I64P = C.POINTER(C.c_int64)
+_FP = C.POINTER(C.c_float)
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+
+
+class FpConfig(C.Structure):
+    _fields_ = [("integration", C.c_int32), ("dt", C.c_float), ("gravity", C.c_float * 3), ("wind", C.c_float * 3),
+                ("lift_coefficient", C.c_float), ("drag_coefficient", C.c_float),
+                ("coefficient_of_restitution", C.c_float), ("coefficient_of_friction", C.c_float)]
+
+
+class FpStrutOverride(C.Structure):
+    _fields_ = [("a", C.c_uint64), ("b", C.c_uint64), ("constant", C.c_float), ("damping", C.c_float)]
+
+
+class FpTorsionConfig(C.Structure):
+    _fields_ = [("spring_constant", C.c_float), ("spring_damping", C.c_float)]
+
+
+SIGNATURES = {
+    "fp_last_error": (C.c_char_p, []),
+    "fp_version": (C.c_char_p, []),
+    "fp_config_default": (None, [C.POINTER(FpConfig)]),
+    "fp_mesh_new": (C.c_int, [_FP, _U64, _U64P, _U64, C.c_float, C.c_float, C.c_float, C.POINTER(FpTorsionConfig),
+                              C.POINTER(FpStrutOverride), _U64, _PP]),
+    "fp_cloth_new": (C.c_int, [_U64, _U64, C.c_float, _FP, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                               C.c_float, C.c_float, _U64P, _U64, _PP]),
+    "fp_cloth_new_in_plane": (C.c_int, [_U64, _U64, C.c_float, _FP, C.c_float, C.c_float, C.c_float, C.c_float,
+                                        C.c_float, C.c_float, C.c_float, _U64P, _U64, C.c_int32, _PP]),
+    "fp_mesh_add_strut": (C.c_int, [_P, _U64, _U64, C.c_float, C.c_float]),
+    "fp_mesh_add_pin": (C.c_int, [_P, _U64]),
+    "fp_mesh_counts": (C.c_int, [_P, _U64P, _U64P, _U64P, _U64P]),
+    "fp_mesh_get_struts": (C.c_int, [_P, _U64P, _FP, _I64P]),
+    "fp_mesh_get_torsion": (C.c_int, [_P, C.POINTER(C.c_uint8), _FP]),
+    "fp_mesh_get_faces": (C.c_int, [_P, _U64P, _U64P]),
+    "fp_mesh_get_pins": (C.c_int, [_P, _U64P]),
+    "fp_mesh_get_points": (C.c_int, [_P, _FP]),
+    "fp_mesh_set_points": (C.c_int, [_P, _FP, _U64]),
+    "fp_mesh_get_vertices": (C.c_int, [_P, _FP, _U64P]),
+    "fp_mesh_free": (None, [_P]),
+    "fp_obstacle_new": (C.c_int, [_FP, _U64, _U64P, _U64, _PP]),
+    "fp_obstacle_counts": (C.c_int, [_P, _U64P, _U64P, _U64P]),
+    "fp_obstacle_free": (None, [_P]),
+    "fp_sim_new": (C.c_int, [_PP, _U64, _PP, _U64, _PP]),
+    "fp_sim_set_config": (C.c_int, [_P, C.POINTER(FpConfig)]),
+    "fp_sim_get_config": (C.c_int, [_P, C.POINTER(FpConfig)]),
+    "fp_sim_set_mesh_config": (C.c_int, [_P, _U64, C.POINTER(FpConfig)]),
+    "fp_sim_step": (C.c_int, [_P, _U64, _FP]),
+    "fp_sim_step_async": (C.c_int, [_P, _U64]),
+    "fp_sim_synchronize": (C.c_int, [_P]),
+    "fp_sim_get_timestep": (C.c_int, [_P, _FP]),
+    "fp_sim_mesh_count": (C.c_int, [_P, _U64P]),
+    "fp_sim_mesh_counts": (C.c_int, [_P, _U64, _U64P, _U64P, _U64P, _U64P]),
+    "fp_sim_get_points": (C.c_int, [_P, _U64, _FP]),
+    "fp_sim_set_points": (C.c_int, [_P, _U64, _FP, _U64]),
+    "fp_sim_get_vertices": (C.c_int, [_P, _U64, _FP, _U64P]),
+    "fp_sim_accumulate_forces": (C.c_int, [_P, _U64]),
+    "fp_sim_update_points": (C.c_int, [_P, _U64, _FP, _U64]),
+    "fp_sim_clear_forces": (C.c_int, [_P, _U64]),
+    "fp_sim_set_kernel": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "fp_sim_time_steps": (C.c_int, [_P, _U64, _FP]),
+    "fp_sim_free": (None, [_P]),
+}
+
+_LIB = None
+
+
+class FpError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class FpIndexError(FpError, IndexError):
+    pass
+
+
+class FpValueError(FpError, ValueError):
+    pass
+
+
+def lib() -> C.CDLL:
+    """The loaded C ABI library; raises if it is not built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C feriphys_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(code: int):
+    if code == FP_OK:
+        return
+    msg = lib().fp_last_error().decode(errors="replace")
+    if code == FP_EINVAL and ("out of range" in msg or "index" in msg):
+        raise FpIndexError(code, msg)
+    if code in (FP_EINVAL, FP_ENONMANIFOLD):
+        raise FpValueError(code, msg)
+    raise FpError(code, msg)

@@ -1,0 +1,223 @@
+// gfx950 kernels of the springy step: one thread per vertex, each vertex gathers its own force
+// (struts in strut order, faces in face order), integrates, resolves obstacle collision, restores
+// pins and writes its new state.  Reads come from `in`, writes go to `out` (ping-pong), so one
+// launch is one Simulation::step for every vertex of every mesh it covers.
+//
+//   csr_kernel  : any SpringyMesh (SpringyMesh::new + add_strut), topology from a vertex-centric
+//                 CSR built on the host.
+//   grid_kernel : meshes built by Cloth::new and left unchanged; struts, rest lengths and faces are
+//                 rebuilt from (row, col) on chip, so the only HBM traffic is the state itself.
+#include "kernels.hpp"
+
+namespace fp {
+
+namespace {
+
+__device__ __forceinline__ void store_state(const SoA& o, uint32_t i, f3 x, f3 v) {
+  o.x[i] = x.x; o.y[i] = x.y; o.z[i] = x.z;
+  o.vx[i] = v.x; o.vy[i] = v.y; o.vz[i] = v.z;
+}
+
+// Integration, collision and pins: the tail of Simulation::step for one vertex
+// (springy/simulation.rs:145-155 -> state.rs, springy_mesh.rs:423-497).
+template <class M>
+__device__ __forceinline__ void finish_step(const StepArgs& A, const StepParams& P, uint32_t i, float m, f3 p, f3 v,
+                                            f3 F, bool pin) {
+  f3 xn, vn;
+  integrate<M>(P, m, p, v, F, xn, vn);
+  int32_t hit = first_hit(A.obs, p, xn, P.dt_d);
+  if (hit >= 0) collide_response(A.obs.faces[hit], P, m, p, v, F, xn, vn);
+  if (pin) { xn = p; vn = v; }  // pinned points keep their pre-step record (:492-496)
+  store_state(A.out, i, xn, vn);
+  if (A.force_init) { A.fx[i] = 0.0f; A.fy[i] = 0.0f; A.fz[i] = 0.0f; }  // clear_forces (:656-660)
+}
+
+template <class M, int MODE>
+__global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, const BlockRange* __restrict__ blocks) {
+  const BlockRange br = blocks[blockIdx.x];
+  const uint32_t i = br.begin + threadIdx.x;
+  if (i >= br.end) return;
+  const StepParams P = A.params[br.mesh];
+  const SoA& S = A.in;
+  const f3 p = ld3(S.x, S.y, S.z, i);
+  const f3 v = ld3(S.vx, S.vy, S.vz, i);
+  const float m = P.uniform_mass ? P.mass : A.mass[i];
+  f3 F = A.force_init ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
+  F = add(F, P.gravity);  // apply_external_point_forces (:527-531)
+  for (uint32_t e = T.s_row[i], ee = T.s_row[i + 1]; e < ee; ++e) {  // apply_strut_forces (:533-551)
+    const uint32_t j = T.s_nbr[e];
+    strut_accumulate<M>(F, p, v, ld3(S.x, S.y, S.z, j), ld3(S.vx, S.vy, S.vz, j), T.s_k[e], T.s_c[e], T.s_len[e]);
+  }
+  for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {  // apply_face_forces (:629-654)
+    const uint32_t ent = T.f_ent[e];
+    const uint32_t f = ent >> 2, corner = ent & 3u;
+    const uint32_t a = T.f_v[3 * f], b = T.f_v[3 * f + 1], c = T.f_v[3 * f + 2];
+    const f3 p0 = ld3(S.x, S.y, S.z, a), p1 = ld3(S.x, S.y, S.z, b), p2 = ld3(S.x, S.y, S.z, c);
+    const f3 sum = face_force<M>(p0, p1, p2, ld3(S.vx, S.vy, S.vz, a), ld3(S.vx, S.vy, S.vz, b),
+                                 ld3(S.vx, S.vy, S.vz, c), P);
+    const float w = corner == 0 ? corner_weight<M>(p0, p1, p2)
+                                : (corner == 1 ? corner_weight<M>(p1, p2, p0) : corner_weight<M>(p2, p0, p1));
+    F = add(F, f3{w * sum.x, w * sum.y, w * sum.z});
+  }
+  const bool pin = pinned(A.pins, i);
+  if (pin) F = f3{0.0f, 0.0f, 0.0f};  // (:522-524)
+  if (MODE == MODE_FORCES) {
+    A.fx[i] = F.x; A.fy[i] = F.y; A.fz[i] = F.z;
+    return;
+  }
+  finish_step<M>(A, P, i, m, p, v, F, pin);
+}
+
+// Grid cloth built by Cloth::new (springy/cloth.rs:20-113).  Vertex (r, c) of an R x C grid.
+// Strut ids follow first appearance over the triangle list (top-left triangles of every quad, then
+// bottom-right ones), then the bending struts; so the incident struts of a vertex, in id order, are:
+//   top-left-created  : shear to (r-1,c+1), tensile to (r-1,c)   [quad (r-1,c)]
+//                       shear to (r+1,c-1), tensile to (r,c-1)   [quad (r,c-1)]
+//                       tensile to (r+1,c), tensile to (r,c+1)   [quad (r,c)]
+//   bottom-right-created (last column verticals, last row horizontals), ids after all of the above:
+//                       (r-1,c) if c=C-1, (r,c-1) if r=R-1, (r+1,c) if c=C-1, (r,c+1) if r=R-1
+//   bending (add_strut, cloth.rs:99-110): (r-2,c), (r,c-2), (r,c+2), (r+2,c)
+// Faces: A(q) = [(r',c'+1), (r'+1,c'), (r',c')] for every quad q in row-major order, then
+// B(q) = [(r',c'+1), (r'+1,c'+1), (r'+1,c')]; a vertex sees A(r-1,c), A(r,c-1), A(r,c),
+// B(r-1,c-1), B(r-1,c), B(r,c-1) in that order, as corners 1, 0, 2, 1, 2, 0.
+// Rest lengths are |p_a - p_b| of the construction grid: one of the three components is exactly 0
+// in either plane, so |d| = sqrt(dX*dX + dY*dY) with dX, dY from the grid coordinates below.
+template <class M, int MODE>
+__global__ void __launch_bounds__(kBlock) grid_kernel(StepArgs A, const ClothView* __restrict__ cloths,
+                                                       const BlockRange* __restrict__ blocks) {
+  const BlockRange br = blocks[blockIdx.x];
+  const uint32_t i = br.begin + threadIdx.x;
+  if (i >= br.end) return;
+  const StepParams P = A.params[br.mesh];
+  const ClothView G = cloths[br.mesh];
+  const SoA& S = A.in;
+  const int R = (int)G.rows, C = (int)G.cols;
+  const int li = (int)(i - G.base);
+  const int r = li / C, c = li - r * C;
+  auto id = [&](int rr, int cc) -> uint32_t { return G.base + (uint32_t)(rr * C + cc); };
+  auto X = [&](int cc) -> float { return ((float)cc * G.spacing - G.cx) + G.p0; };
+  auto Y = [&](int rr) -> float { return ((float)rr * G.spacing - G.cy) + G.p1; };
+  const f3 p = ld3(S.x, S.y, S.z, i);
+  const f3 v = ld3(S.vx, S.vy, S.vz, i);
+  const float m = P.uniform_mass ? P.mass : A.mass[i];
+  f3 F = A.force_init ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
+  F = add(F, P.gravity);
+  const float xc = X(c), yr = Y(r);
+  auto strut = [&](int rr, int cc, float kn, float cn) {
+    const float dX = xc - X(cc), dY = yr - Y(rr);
+    const float L = __builtin_sqrtf(dX * dX + dY * dY);
+    const float inv = 1.0f / L;  // Strut::new: NOMINAL_STRUT_LENGTH / length (springy_mesh.rs:83-84)
+    const uint32_t j = id(rr, cc);
+    strut_accumulate<M>(F, p, v, ld3(S.x, S.y, S.z, j), ld3(S.vx, S.vy, S.vz, j), inv * kn, inv * cn, L);
+  };
+  if (r >= 1 && c <= C - 2) { strut(r - 1, c + 1, G.sk, G.sc); strut(r - 1, c, G.tk, G.tc); }
+  if (r <= R - 2 && c >= 1) { strut(r + 1, c - 1, G.sk, G.sc); strut(r, c - 1, G.tk, G.tc); }
+  if (r <= R - 2 && c <= C - 2) { strut(r + 1, c, G.tk, G.tc); strut(r, c + 1, G.tk, G.tc); }
+  if (c == C - 1 && r >= 1) strut(r - 1, c, G.tk, G.tc);
+  if (r == R - 1 && c >= 1) strut(r, c - 1, G.tk, G.tc);
+  if (c == C - 1 && r <= R - 2) strut(r + 1, c, G.tk, G.tc);
+  if (r == R - 1 && c <= C - 2) strut(r, c + 1, G.tk, G.tc);
+  if (r >= 2 && c <= C - 3) strut(r - 2, c, G.bk, G.bc);
+  if (c >= 2 && r <= R - 3) strut(r, c - 2, G.bk, G.bc);
+  if (r <= R - 3 && c <= C - 3) { strut(r, c + 2, G.bk, G.bc); strut(r + 2, c, G.bk, G.bc); }
+
+  auto P3 = [&](int rr, int cc) { return ld3(S.x, S.y, S.z, id(rr, cc)); };
+  auto V3 = [&](int rr, int cc) { return ld3(S.vx, S.vy, S.vz, id(rr, cc)); };
+  // corner 0/1/2 of face (q0, q1, q2): weight = angle at that corner / PI
+  auto face = [&](int r0, int c0, int r1, int c1, int r2, int c2, int corner) {
+    const f3 p0 = P3(r0, c0), p1 = P3(r1, c1), p2 = P3(r2, c2);
+    const f3 sum = face_force<M>(p0, p1, p2, V3(r0, c0), V3(r1, c1), V3(r2, c2), P);
+    const float w = corner == 0 ? corner_weight<M>(p0, p1, p2)
+                                : (corner == 1 ? corner_weight<M>(p1, p2, p0) : corner_weight<M>(p2, p0, p1));
+    F = add(F, f3{w * sum.x, w * sum.y, w * sum.z});
+  };
+  if (r >= 1 && c <= C - 2) face(r - 1, c + 1, r, c, r - 1, c, 1);          // A(r-1, c)
+  if (r <= R - 2 && c >= 1) face(r, c, r + 1, c - 1, r, c - 1, 0);          // A(r, c-1)
+  if (r <= R - 2 && c <= C - 2) face(r, c + 1, r + 1, c, r, c, 2);          // A(r, c)
+  if (r >= 1 && c >= 1) face(r - 1, c, r, c, r, c - 1, 1);                  // B(r-1, c-1)
+  if (r >= 1 && c <= C - 2) face(r - 1, c + 1, r, c + 1, r, c, 2);          // B(r-1, c)
+  if (r <= R - 2 && c >= 1) face(r, c, r + 1, c, r + 1, c - 1, 0);          // B(r, c-1)
+
+  const bool pin = pinned(A.pins, i);
+  if (pin) F = f3{0.0f, 0.0f, 0.0f};
+  if (MODE == MODE_FORCES) {
+    A.fx[i] = F.x; A.fy[i] = F.y; A.fz[i] = F.z;
+    return;
+  }
+  finish_step<M>(A, P, i, m, p, v, F, pin);
+}
+
+// SpringyMesh::update_points (springy_mesh.rs:423-497) for records supplied by the caller.
+__global__ void __launch_bounds__(kBlock) update_points_kernel(StepArgs A, const float* __restrict__ rec,
+                                                                uint32_t rec_base, float* __restrict__ mass_out,
+                                                                const BlockRange* __restrict__ blocks) {
+  const BlockRange br = blocks[blockIdx.x];
+  const uint32_t i = br.begin + threadIdx.x;
+  if (i >= br.end) return;
+  const StepParams P = A.params[br.mesh];
+  const float* q = rec + 10ull * (i - rec_base);
+  const float m_old = P.uniform_mass ? P.mass : A.mass[i];
+  const f3 p = ld3(A.in.x, A.in.y, A.in.z, i), v = ld3(A.in.vx, A.in.vy, A.in.vz, i);
+  const f3 F = f3{A.fx[i], A.fy[i], A.fz[i]};
+  f3 xn = f3{q[1], q[2], q[3]}, vn = f3{q[4], q[5], q[6]};
+  float mn = q[0];
+  f3 Fn = f3{q[7], q[8], q[9]};
+  const int32_t hit = first_hit(A.obs, p, xn, P.dt_d);
+  if (hit >= 0) collide_response(A.obs.faces[hit], P, m_old, p, v, F, xn, vn);
+  if (pinned(A.pins, i)) { xn = p; vn = v; mn = m_old; Fn = F; }
+  store_state(A.in, i, xn, vn);
+  mass_out[i] = mn;
+  A.fx[i] = Fn.x; A.fy[i] = Fn.y; A.fz[i] = Fn.z;
+}
+
+__global__ void fill_kernel(float* p, uint64_t n, float v) {
+  for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
+    p[k] = v;
+}
+
+}  // namespace
+
+hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
+                      bool exact, hipStream_t s) {
+  if (n_blocks == 0) return hipSuccess;
+  dim3 g(n_blocks), b(kBlock);
+  if (exact) {
+    if (mode == MODE_STEP) csr_kernel<ExactMath, MODE_STEP><<<g, b, 0, s>>>(a, t, blocks);
+    else csr_kernel<ExactMath, MODE_FORCES><<<g, b, 0, s>>>(a, t, blocks);
+  } else {
+    if (mode == MODE_STEP) csr_kernel<FastMath, MODE_STEP><<<g, b, 0, s>>>(a, t, blocks);
+    else csr_kernel<FastMath, MODE_FORCES><<<g, b, 0, s>>>(a, t, blocks);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_grid(const StepArgs& a, const ClothView* cloths, const BlockRange* blocks, uint32_t n_blocks,
+                       int mode, bool exact, hipStream_t s) {
+  if (n_blocks == 0) return hipSuccess;
+  dim3 g(n_blocks), b(kBlock);
+  if (exact) {
+    if (mode == MODE_STEP) grid_kernel<ExactMath, MODE_STEP><<<g, b, 0, s>>>(a, cloths, blocks);
+    else grid_kernel<ExactMath, MODE_FORCES><<<g, b, 0, s>>>(a, cloths, blocks);
+  } else {
+    if (mode == MODE_STEP) grid_kernel<FastMath, MODE_STEP><<<g, b, 0, s>>>(a, cloths, blocks);
+    else grid_kernel<FastMath, MODE_FORCES><<<g, b, 0, s>>>(a, cloths, blocks);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_update_points(const StepArgs& a, const float* nw_rec, uint32_t rec_base, float* mass_out,
+                                const BlockRange* blocks, uint32_t n_blocks, hipStream_t s) {
+  if (n_blocks == 0) return hipSuccess;
+  update_points_kernel<<<dim3(n_blocks), dim3(kBlock), 0, s>>>(a, nw_rec, rec_base, mass_out, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  fill_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+}  // namespace fp

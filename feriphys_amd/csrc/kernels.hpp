@@ -1,0 +1,62 @@
+// Launch interface of the gfx950 springy-step kernels (kernels.hip).
+#pragma once
+
+#include "step_device.hpp"
+
+namespace fp {
+
+struct SoA {
+  float *x, *y, *z, *vx, *vy, *vz;
+};
+
+// One workgroup's share of one mesh: vertices [begin, end) of mesh `mesh`.
+struct BlockRange {
+  uint32_t mesh, begin, end, pad;
+};
+
+// Vertex-centric CSR of a packed set of meshes (global vertex and face ids).
+struct CsrView {
+  const uint32_t* s_row;  // N+1: incident struts of each vertex, in strut-index order
+  const uint32_t* s_nbr;
+  const float* s_len;
+  const float* s_k;
+  const float* s_c;
+  const uint32_t* f_row;  // N+1: incident faces of each vertex, in face-index order
+  const uint32_t* f_ent;  // (face << 2) | corner
+  const uint32_t* f_v;    // 3 global vertex ids per face
+};
+
+// A Cloth::new grid: enough to rebuild its struts, rest lengths and faces on chip.
+struct ClothView {
+  uint32_t base, rows, cols, pad;
+  float spacing, cx, cy, p0, p1;  // construction grid: X(c) = (c*s - cx) + p0, Y(r) = (r*s - cy) + p1
+  float tk, tc, sk, sc, bk, bc;
+};
+
+struct StepArgs {
+  SoA in, out;
+  float *fx, *fy, *fz;  // force records (split API / non-zero initial force)
+  const float* mass;    // per-vertex mass (used when a mesh's mass is not uniform)
+  const uint32_t* pins; // pin bitmask over global vertex ids
+  const StepParams* params;
+  ObsView obs;
+  int32_t force_init;   // 1: start from the force records and clear them after the step
+};
+
+enum StepMode { MODE_STEP = 0, MODE_FORCES = 1 };
+
+hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
+                      bool exact, hipStream_t s);
+hipError_t launch_grid(const StepArgs& a, const ClothView* cloths, const BlockRange* blocks, uint32_t n_blocks,
+                       int mode, bool exact, hipStream_t s);
+// SpringyMesh::update_points on device: old state = a.in (+ force records, masses), new = `nw_rec`
+// (10-float Stateful records, indexed by global vertex id); the result is written in place into
+// a.in, the force records and the masses.
+hipError_t launch_update_points(const StepArgs& a, const float* nw_rec, uint32_t rec_base, float* mass_out,
+                                const BlockRange* blocks,
+                                uint32_t n_blocks, hipStream_t s);
+hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s);
+
+constexpr uint32_t kBlock = 256;
+
+}  // namespace fp

@@ -1,0 +1,891 @@
+// C ABI of the gfx950 springy step: host handles, device packing, kernel dispatch.
+//
+// Device layout of a simulation (all meshes packed, global vertex ids = mesh base + local id):
+//   state[2][6]   SoA f32 x, y, z, vx, vy, vz, ping-ponged every step (24 B read + 24 B written
+//                 per vertex-step: the whole HBM traffic of the grid kernel)
+//   force[3]      f32 Fx, Fy, Fz records, touched only by the split API (accumulate_forces /
+//                 update_points) or when a step starts from non-zero forces
+//   mass          f32 per vertex (read only for meshes whose masses are not uniform)
+//   pins          1 bit per vertex
+//   CSR           per vertex: incident struts (neighbour, rest length, k, c) in strut order and
+//                 incident faces ((face << 2) | corner) in face order; 3 vertex ids per face
+//   obstacles     faces (corners, unit normal, dropped axis) + a uniform grid over them
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/feriphys_gpu.h"
+#include "kernels.hpp"
+#include "mesh_host.hpp"
+
+struct fp_mesh {
+  fp::HostMesh m;
+};
+struct fp_obstacle {
+  fp::HostObstacle o;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+int fail(const fp::Status& s) { return fail(s.code, s.msg); }
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(FP_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T> struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, count * sizeof(T));
+  }
+  hipError_t upload(const T* h, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
+  }
+};
+
+// Duration::from_secs_f32(dt).as_secs_f32() (springy_mesh.rs:442; springy/simulation.rs:158).
+// The float -> Duration conversion is exact with round-half-to-even to whole nanoseconds: dt has
+// a 24-bit significand and 1e9 a 21-bit odd part, so dt * 1e9 is exact in x87 long double and
+// nearbyintl rounds it half-to-even.  Then secs as f32 + (nanos as f32) / 1e9.
+bool duration_roundtrip(float dt, float& out) {
+  if (!(dt >= 0.0f) || !std::isfinite(dt)) return false;
+  long double ns = (long double)dt * 1000000000.0L;
+  long double r = nearbyintl(ns);
+  if (r > 1.8e19L) return false;
+  uint64_t total = (uint64_t)r;
+  uint64_t secs = total / 1000000000ull;
+  uint32_t nanos = (uint32_t)(total % 1000000000ull);
+  out = (float)secs + (float)nanos / 1000000000.0f;
+  return true;
+}
+
+fp_config default_config() {
+  fp_config c;
+  c.integration = FP_RK4;
+  c.dt = 0.001f;  // Duration::from_millis(1).as_secs_f32()
+  c.gravity[0] = 0.0f; c.gravity[1] = -10.0f; c.gravity[2] = 0.0f;
+  c.wind[0] = c.wind[1] = c.wind[2] = 0.0f;
+  c.lift_coefficient = 1.0f;
+  c.drag_coefficient = 1.0f;
+  c.coefficient_of_restitution = 0.95f;
+  c.coefficient_of_friction = 0.3f;
+  return c;
+}
+
+struct MeshSlot {
+  fp::HostMesh host;
+  uint64_t vbase = 0, fbase = 0;
+  fp_config cfg;
+  fp::StepParams params;
+  bool uniform_mass = true;
+  bool use_grid = false;
+  uint32_t blk_begin = 0, blk_end = 0;
+};
+
+}  // namespace
+
+struct fp_sim {
+  std::vector<MeshSlot> meshes;
+  uint64_t N = 0, F = 0;
+  hipStream_t stream = nullptr;
+  int cur = 0;
+  DBuf<float> state[2][6];
+  DBuf<float> force[3];
+  bool forces_dirty = false;
+  DBuf<float> mass;
+  DBuf<uint32_t> pins;
+  DBuf<uint32_t> s_row, s_nbr, f_row, f_ent, f_v;
+  DBuf<float> s_len, s_k, s_c;
+  DBuf<fp::ObsFace> obs_faces;
+  DBuf<uint32_t> cell_start[4], cell_faces[4], always;
+  uint64_t n_always = 0, n_binned[4] = {0, 0, 0, 0};
+  fp::ObsView obs{};
+  uint64_t n_obs_vertices = 0, n_obs_faces = 0;
+  DBuf<fp::BlockRange> all_blocks, grid_blocks, csr_blocks;
+  std::vector<fp::BlockRange> h_blocks;
+  uint32_t n_grid_blocks = 0, n_csr_blocks = 0;
+  DBuf<fp::ClothView> cloths;
+  DBuf<fp::StepParams> params;
+  int32_t kernel = FP_KERNEL_AUTO, math = FP_MATH_EXACT;
+  DBuf<float> tmp_rec;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+  ~fp_sim() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  fp::StepArgs args(bool force_init) {
+    fp::StepArgs a{};
+    a.in = fp::SoA{state[cur][0].p, state[cur][1].p, state[cur][2].p, state[cur][3].p, state[cur][4].p, state[cur][5].p};
+    int o = cur ^ 1;
+    a.out = fp::SoA{state[o][0].p, state[o][1].p, state[o][2].p, state[o][3].p, state[o][4].p, state[o][5].p};
+    a.fx = force[0].p;
+    a.fy = force[1].p;
+    a.fz = force[2].p;
+    a.mass = mass.p;
+    a.pins = pins.p;
+    a.params = params.p;
+    a.obs = obs;
+    a.force_init = force_init ? 1 : 0;
+    return a;
+  }
+  fp::CsrView csr() const {
+    return fp::CsrView{s_row.p, s_nbr.p, s_len.p, s_k.p, s_c.p, f_row.p, f_ent.p, f_v.p};
+  }
+};
+
+namespace {
+
+fp::StepParams make_params(const fp_config& c, float mass, bool uniform) {
+  fp::StepParams P{};
+  P.integration = c.integration;
+  P.dt = c.dt;
+  P.dt_half = c.dt * 0.5f;
+  P.dt6 = c.dt / 6.0f;
+  P.dt3 = c.dt / 3.0f;
+  float dd = 0.0f;
+  duration_roundtrip(c.dt, dd);
+  P.dt_d = dd;
+  P.gravity = fp::f3{c.gravity[0], c.gravity[1], c.gravity[2]};
+  P.wind = fp::f3{c.wind[0], c.wind[1], c.wind[2]};
+  P.neg_drag = -1.0f * c.drag_coefficient;
+  P.neg_lift = -1.0f * c.lift_coefficient;
+  P.restitution = c.coefficient_of_restitution;
+  P.friction = c.coefficient_of_friction;
+  P.mass = mass;
+  P.uniform_mass = uniform ? 1 : 0;
+  return P;
+}
+
+int check_config(const fp_config* c) {
+  if (!c) return fail(FP_EINVAL, "config is NULL");
+  if (c->integration != FP_EULER && c->integration != FP_RK4) return fail(FP_EINVAL, "unknown integration");
+  float dd;
+  if (!duration_roundtrip(c->dt, dd)) return fail(FP_EINVAL, "dt must be finite and non-negative (Duration)");
+  return FP_OK;
+}
+
+// Re-derives whether the mesh's masses are all equal (the grid kernel and the scalar-mass path
+// need that) from its records.
+void refresh_mass(MeshSlot& ms, const float* rec, uint64_t n) {
+  bool uni = true;
+  float m0 = n ? rec[0] : 0.0f;
+  for (uint64_t i = 1; i < n && uni; ++i) uni = (rec[10 * i] == m0) && !(std::signbit(rec[10 * i]) != std::signbit(m0));
+  ms.uniform_mass = uni;
+  ms.params = make_params(ms.cfg, m0, uni);
+}
+
+int upload_params(fp_sim* s) {
+  std::vector<fp::StepParams> p;
+  for (auto& m : s->meshes) p.push_back(m.params);
+  HIP_TRY(s->params.upload(p.data(), p.size()));
+  return FP_OK;
+}
+
+int build_block_lists(fp_sim* s) {
+  std::vector<fp::BlockRange> g, c;
+  for (auto& ms : s->meshes) {
+    bool grid = ms.host.cloth.valid && ms.uniform_mass && s->kernel != FP_KERNEL_CSR;
+    ms.use_grid = grid;
+    for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) (grid ? g : c).push_back(s->h_blocks[b]);
+  }
+  s->n_grid_blocks = (uint32_t)g.size();
+  s->n_csr_blocks = (uint32_t)c.size();
+  HIP_TRY(s->grid_blocks.upload(g.data(), g.size()));
+  HIP_TRY(s->csr_blocks.upload(c.data(), c.size()));
+  return FP_OK;
+}
+
+// Obstacle faces and their broad-phase grids (see ObsView).  Faces whose dropped normal component
+// is at least kTau are binned in 3D; those between kThin and kTau in a 2D grid of their dropped
+// axis; the rest are tested for every vertex.  Boxes are grown by kMargin, far above the rounding
+// of the crossing point and of the containment test at the coordinates these scenes use.
+constexpr float kMargin = 1.0e-2f;
+constexpr float kTau = 0.05f;
+constexpr float kThin = 1.0e-3f;
+
+int build_grid(fp_sim* s, int g, const std::vector<fp::ObsFace>& faces, const std::vector<uint32_t>& ids,
+               int collapse) {
+  fp::GridView& G = s->obs.grid[g];
+  G = fp::GridView{};
+  if (ids.empty()) return FP_OK;
+  auto box = [&](const fp::ObsFace& q, int k, float& l, float& h) {
+    l = std::min(q.v0[k], std::min(q.v1[k], q.v2[k])) - kMargin;
+    h = std::max(q.v0[k], std::max(q.v1[k], q.v2[k])) + kMargin;
+  };
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double ext_sum = 0.0;
+  int n_axes = 0;
+  for (uint32_t f : ids)
+    for (int k = 0; k < 3; ++k) {
+      if (k == collapse) continue;
+      float l, h;
+      box(faces[f], k, l, h);
+      lo[k] = std::min(lo[k], l);
+      hi[k] = std::max(hi[k], h);
+      ext_sum += (double)(h - l);
+    }
+  for (int k = 0; k < 3; ++k) n_axes += k != collapse;
+  const double mean_ext = ext_sum / ((double)n_axes * (double)ids.size());
+  for (int k = 0; k < 3; ++k) {
+    if (k == collapse) {
+      G.lo[k] = 0.0f;
+      G.inv_cell[k] = 0.0f;
+      G.dims[k] = 1;
+      continue;
+    }
+    const double ext = std::max((double)(hi[k] - lo[k]), 1e-6);
+    const double cell = std::max(mean_ext, ext / 128.0);
+    G.dims[k] = (int)std::min(128.0, std::max(1.0, std::ceil(ext / cell)));
+    G.inv_cell[k] = (float)((double)G.dims[k] / ext);
+    G.lo[k] = lo[k];
+  }
+  const uint64_t n_cells = (uint64_t)G.dims[0] * G.dims[1] * G.dims[2];
+  // the kernel's mapping from a coordinate to a cell, with the same f32 operations
+  auto range = [&](const fp::ObsFace& q, int k, int& c0, int& c1) {
+    float l, h;
+    box(q, k, l, h);
+    const float fl = (l - G.lo[k]) * G.inv_cell[k];
+    const float fh = (h - G.lo[k]) * G.inv_cell[k];
+    c0 = fl < 0.0f ? 0 : std::min((int)fl, G.dims[k] - 1);
+    c1 = fh >= (float)G.dims[k] ? G.dims[k] - 1 : std::max(0, (int)fh);
+  };
+  std::vector<uint32_t> start(n_cells + 1, 0);
+  auto visit = [&](auto&& fn) {
+    for (uint32_t f : ids) {  // ascending face ids -> ascending cell lists
+      int r[3][2];
+      for (int k = 0; k < 3; ++k) range(faces[f], k, r[k][0], r[k][1]);
+      for (int z = r[2][0]; z <= r[2][1]; ++z)
+        for (int y = r[1][0]; y <= r[1][1]; ++y)
+          for (int x = r[0][0]; x <= r[0][1]; ++x) fn(((uint64_t)z * G.dims[1] + y) * G.dims[0] + x, f);
+    }
+  };
+  visit([&](uint64_t c, uint32_t) { start[c + 1]++; });
+  for (uint64_t c = 0; c < n_cells; ++c) start[c + 1] += start[c];
+  std::vector<uint32_t> fill(start.begin(), start.end() - 1), lists(start[n_cells]);
+  visit([&](uint64_t c, uint32_t f) { lists[fill[c]++] = f; });
+  HIP_TRY(s->cell_start[g].upload(start.data(), start.size()));
+  HIP_TRY(s->cell_faces[g].upload(lists.data(), lists.size()));
+  G.cell_start = s->cell_start[g].p;
+  G.cell_faces = s->cell_faces[g].p;
+  return FP_OK;
+}
+
+int upload_obstacles(fp_sim* s, const std::vector<fp::HostObstacle>& obstacles) {
+  std::vector<fp::ObsFace> faces;
+  for (auto& o : obstacles) {
+    s->n_obs_vertices += o.vertices.size() / 3;
+    for (uint64_t t = 0; t < o.n_faces(); ++t) {
+      const float* q = &o.faces[9 * t];
+      fp::ObsFace f{};
+      for (int k = 0; k < 3; ++k) { f.v0[k] = q[k]; f.v1[k] = q[3 + k]; f.v2[k] = q[6 + k]; }
+      fp::f3 a{q[0], q[1], q[2]}, b{q[3], q[4], q[5]}, c{q[6], q[7], q[8]};
+      fp::f3 n = fp::h_normalize(fp::cross(fp::sub(b, a), fp::sub(c, a)));  // Face::normal (collidable_mesh.rs:41-43)
+      f.n[0] = n.x; f.n[1] = n.y; f.n[2] = n.z;
+      // collidable_mesh.rs:133-154: signed comparisons, x first, then y, else z
+      f.axis = (n.x >= n.y && n.x >= n.z) ? 0 : ((n.y >= n.x && n.y >= n.z) ? 1 : 2);
+      faces.push_back(f);
+    }
+  }
+  s->n_obs_faces = faces.size();
+  HIP_TRY(s->obs_faces.upload(faces.data(), faces.size()));
+  fp::ObsView& O = s->obs;
+  O = fp::ObsView{};
+  O.faces = s->obs_faces.p;
+  O.n_faces = (uint32_t)faces.size();
+  O.margin = kMargin;
+  if (faces.size() <= 8) return FP_OK;  // scan every face in order
+
+  std::vector<uint32_t> always, ids[4];
+  for (uint32_t f = 0; f < faces.size(); ++f) {
+    const fp::ObsFace& q = faces[f];
+    const float an = std::fabs(q.n[q.axis]);
+    bool finite = true;
+    for (int k = 0; k < 3; ++k)
+      finite = finite && std::isfinite(q.v0[k]) && std::isfinite(q.v1[k]) && std::isfinite(q.v2[k]);
+    if (!finite || !(an >= kThin)) always.push_back(f);
+    else if (an >= kTau) ids[0].push_back(f);
+    else ids[1 + q.axis].push_back(f);
+  }
+  O.use_grid = 1;
+  s->n_always = always.size();
+  for (int g = 0; g < 4; ++g) s->n_binned[g] = ids[g].size();
+  HIP_TRY(s->always.upload(always.data(), always.size()));
+  O.always = s->always.p;
+  O.n_always = (uint32_t)always.size();
+  for (int g = 0; g < 4; ++g)
+    if (int e = build_grid(s, g, faces, ids[g], g == 0 ? -1 : g - 1)) return e;
+  return FP_OK;
+}
+
+int check_mesh(const fp_mesh* m) {
+  if (!m) return fail(FP_EINVAL, "mesh is NULL");
+  if (m->m.moved) return fail(FP_ESTATE, "mesh was moved into a simulation");
+  return FP_OK;
+}
+
+int check_sim_mesh(const fp_sim* s, uint64_t mesh) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (mesh >= s->meshes.size()) return fail(FP_EINVAL, "mesh index out of range");
+  return FP_OK;
+}
+
+int run_steps(fp_sim* s, uint64_t n) {
+  for (uint64_t k = 0; k < n; ++k) {
+    fp::StepArgs a = s->args(s->forces_dirty);
+    bool exact = s->math == FP_MATH_EXACT;
+    HIP_TRY(fp::launch_grid(a, s->cloths.p, s->grid_blocks.p, s->n_grid_blocks, fp::MODE_STEP, exact, s->stream));
+    HIP_TRY(fp::launch_csr(a, s->csr(), s->csr_blocks.p, s->n_csr_blocks, fp::MODE_STEP, exact, s->stream));
+    s->cur ^= 1;
+    s->forces_dirty = false;
+  }
+  return FP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fp_last_error(void) { return g_err.c_str(); }
+const char* fp_version(void) { return "feriphys_gpu 0.1 gfx950"; }
+void fp_config_default(fp_config* out) {
+  if (out) *out = default_config();
+}
+
+int fp_mesh_new(const float* positions, uint64_t n_vertices, const uint64_t* indices, uint64_t n_indices, float mass,
+                float k, float c, const fp_torsion_config* torsion, const fp_strut_override* overrides,
+                uint64_t n_overrides, fp_mesh** out) {
+  if (!out) return fail(FP_EINVAL, "out is NULL");
+  *out = nullptr;
+  std::unique_ptr<fp_mesh> m(new (std::nothrow) fp_mesh());
+  if (!m) return fail(FP_ENOMEM, "out of host memory");
+  try {
+    fp::Status st = fp::build_springy_mesh(positions, n_vertices, indices, n_indices, mass, k, c, torsion, overrides,
+                                           n_overrides, m->m);
+    if (st.code != FP_OK) return fail(st);
+  } catch (const std::bad_alloc&) {
+    return fail(FP_ENOMEM, "out of host memory");
+  }
+  *out = m.release();
+  return FP_OK;
+}
+
+int fp_cloth_new_in_plane(uint64_t rows, uint64_t cols, float spacing, const float position[3], float point_mass,
+                          float tk, float tc, float sk, float sc, float bk, float bc, const uint64_t* pins,
+                          uint64_t n_pins, int32_t plane, fp_mesh** out) {
+  if (!out) return fail(FP_EINVAL, "out is NULL");
+  *out = nullptr;
+  std::unique_ptr<fp_mesh> m(new (std::nothrow) fp_mesh());
+  if (!m) return fail(FP_ENOMEM, "out of host memory");
+  try {
+    fp::Status st = fp::build_cloth(rows, cols, spacing, position, point_mass, tk, tc, sk, sc, bk, bc, pins, n_pins,
+                                    plane, m->m);
+    if (st.code != FP_OK) return fail(st);
+  } catch (const std::bad_alloc&) {
+    return fail(FP_ENOMEM, "out of host memory");
+  }
+  *out = m.release();
+  return FP_OK;
+}
+
+int fp_cloth_new(uint64_t rows, uint64_t cols, float spacing, const float position[3], float point_mass, float tk,
+                 float tc, float sk, float sc, float bk, float bc, const uint64_t* pins, uint64_t n_pins,
+                 fp_mesh** out) {
+  return fp_cloth_new_in_plane(rows, cols, spacing, position, point_mass, tk, tc, sk, sc, bk, bc, pins, n_pins,
+                               FP_PLANE_XY, out);
+}
+
+int fp_mesh_add_strut(fp_mesh* mesh, uint64_t v0, uint64_t v1, float k, float c) {
+  if (int e = check_mesh(mesh)) return e;
+  fp::Status st = fp::mesh_add_strut(mesh->m, v0, v1, k, c);
+  return st.code == FP_OK ? FP_OK : fail(st);
+}
+
+int fp_mesh_add_pin(fp_mesh* mesh, uint64_t index) {
+  if (int e = check_mesh(mesh)) return e;
+  mesh->m.pins.push_back(index);
+  return FP_OK;
+}
+
+int fp_mesh_counts(const fp_mesh* mesh, uint64_t* np, uint64_t* ns, uint64_t* nf, uint64_t* npin) {
+  if (int e = check_mesh(mesh)) return e;
+  if (np) *np = mesh->m.n;
+  if (ns) *ns = mesh->m.n_struts();
+  if (nf) *nf = mesh->m.n_faces();
+  if (npin) *npin = mesh->m.pins.size();
+  return FP_OK;
+}
+
+static void copy_struts(const fp::HostMesh& m, uint64_t* v01, float* kcl, int64_t* faces) {
+  for (uint64_t s = 0; s < m.n_struts(); ++s) {
+    if (v01) { v01[2 * s] = m.s_v0[s]; v01[2 * s + 1] = m.s_v1[s]; }
+    if (kcl) { kcl[3 * s] = m.s_k[s]; kcl[3 * s + 1] = m.s_c[s]; kcl[3 * s + 2] = m.s_len[s]; }
+    if (faces) { faces[2 * s] = m.s_f0[s]; faces[2 * s + 1] = m.s_f1[s]; }
+  }
+}
+
+int fp_mesh_get_struts(const fp_mesh* mesh, uint64_t* v01, float* kcl, int64_t* faces) {
+  if (int e = check_mesh(mesh)) return e;
+  copy_struts(mesh->m, v01, kcl, faces);
+  return FP_OK;
+}
+
+int fp_mesh_get_torsion(const fp_mesh* mesh, uint8_t* has, float* kcr) {
+  if (int e = check_mesh(mesh)) return e;
+  const fp::HostMesh& m = mesh->m;
+  for (uint64_t s = 0; s < m.n_struts(); ++s) {
+    if (has) has[s] = m.t_has[s];
+    if (kcr) { kcr[3 * s] = m.t_k[s]; kcr[3 * s + 1] = m.t_c[s]; kcr[3 * s + 2] = m.t_rest[s]; }
+  }
+  return FP_OK;
+}
+
+int fp_mesh_get_faces(const fp_mesh* mesh, uint64_t* s3, uint64_t* v3) {
+  if (int e = check_mesh(mesh)) return e;
+  const fp::HostMesh& m = mesh->m;
+  for (uint64_t k = 0; k < m.f_v.size(); ++k) {
+    if (s3) s3[k] = m.f_s[k];
+    if (v3) v3[k] = m.f_v[k];
+  }
+  return FP_OK;
+}
+
+int fp_mesh_get_pins(const fp_mesh* mesh, uint64_t* pins) {
+  if (int e = check_mesh(mesh)) return e;
+  if (pins) std::copy(mesh->m.pins.begin(), mesh->m.pins.end(), pins);
+  return FP_OK;
+}
+
+int fp_mesh_get_points(const fp_mesh* mesh, float* rec) {
+  if (int e = check_mesh(mesh)) return e;
+  if (!rec) return fail(FP_EINVAL, "records is NULL");
+  std::copy(mesh->m.rec.begin(), mesh->m.rec.end(), rec);
+  return FP_OK;
+}
+
+int fp_mesh_set_points(fp_mesh* mesh, const float* rec, uint64_t n) {
+  if (int e = check_mesh(mesh)) return e;
+  if (n != mesh->m.n) return fail(FP_EINVAL, "State Vector incorrect size! (record count != point count)");
+  if (n && !rec) return fail(FP_EINVAL, "records is NULL");
+  std::copy(rec, rec + 10 * n, mesh->m.rec.begin());
+  return FP_OK;
+}
+
+static void copy_vertices(const fp::HostMesh& m, const float* rec, float* pos, uint64_t* idx) {
+  if (pos)
+    for (uint64_t i = 0; i < m.n; ++i)
+      for (int k = 0; k < 3; ++k) pos[3 * i + k] = rec[10 * i + 1 + k];
+  if (idx)
+    for (uint64_t k = 0; k < m.f_v.size(); ++k) idx[k] = m.f_v[k];
+}
+
+int fp_mesh_get_vertices(const fp_mesh* mesh, float* pos, uint64_t* idx) {
+  if (int e = check_mesh(mesh)) return e;
+  copy_vertices(mesh->m, mesh->m.rec.data(), pos, idx);
+  return FP_OK;
+}
+
+void fp_mesh_free(fp_mesh* mesh) { delete mesh; }
+
+int fp_obstacle_new(const float* positions, uint64_t n, const uint64_t* indices, uint64_t n_idx, fp_obstacle** out) {
+  if (!out) return fail(FP_EINVAL, "out is NULL");
+  *out = nullptr;
+  std::unique_ptr<fp_obstacle> o(new (std::nothrow) fp_obstacle());
+  if (!o) return fail(FP_ENOMEM, "out of host memory");
+  fp::Status st = fp::build_obstacle(positions, n, indices, n_idx, o->o);
+  if (st.code != FP_OK) return fail(st);
+  *out = o.release();
+  return FP_OK;
+}
+
+int fp_obstacle_counts(const fp_obstacle* o, uint64_t* nv, uint64_t* ne, uint64_t* nf) {
+  if (!o) return fail(FP_EINVAL, "obstacle is NULL");
+  if (o->o.moved) return fail(FP_ESTATE, "obstacle was moved into a simulation");
+  if (nv) *nv = o->o.vertices.size() / 3;
+  if (ne) *ne = o->o.n_edges;
+  if (nf) *nf = o->o.n_faces();
+  return FP_OK;
+}
+
+void fp_obstacle_free(fp_obstacle* o) { delete o; }
+
+int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* obstacles, uint64_t n_obstacles,
+               fp_sim** out) {
+  // Ownership moves in, as Simulation::new takes the Vecs by value: free the handles on every path.
+  std::vector<std::unique_ptr<fp_mesh>> ms;
+  std::vector<std::unique_ptr<fp_obstacle>> os;
+  for (uint64_t i = 0; i < n_meshes; ++i)
+    if (meshes && meshes[i]) ms.emplace_back(meshes[i]);
+  for (uint64_t i = 0; i < n_obstacles; ++i)
+    if (obstacles && obstacles[i]) os.emplace_back(obstacles[i]);
+  if (!out) return fail(FP_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (ms.size() != n_meshes || os.size() != n_obstacles) return fail(FP_EINVAL, "NULL mesh or obstacle handle");
+  for (auto& m : ms)
+    if (m->m.moved) return fail(FP_ESTATE, "mesh was moved into a simulation");
+  int dev_count = 0;
+  if (hipGetDeviceCount(&dev_count) != hipSuccess || dev_count == 0) return fail(FP_ENODEV, "no HIP device");
+
+  std::unique_ptr<fp_sim> s(new (std::nothrow) fp_sim());
+  if (!s) return fail(FP_ENOMEM, "out of host memory");
+  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  HIP_TRY(hipEventCreate(&s->ev0));
+  HIP_TRY(hipEventCreate(&s->ev1));
+  const fp_config cfg0 = default_config();
+  uint64_t N = 0, F = 0, Sdir = 0, Fent = 0;
+  for (auto& m : ms) {
+    for (uint64_t p : m->m.pins)
+      if (p >= m->m.n) return fail(FP_EINVAL, "pin index out of range");  // springy_mesh.rs:523 panics
+    MeshSlot slot;
+    slot.host = std::move(m->m);
+    slot.vbase = N;
+    slot.fbase = F;
+    slot.cfg = cfg0;
+    N += slot.host.n;
+    F += slot.host.n_faces();
+    Sdir += 2 * slot.host.n_struts();
+    Fent += 3 * slot.host.n_faces();
+    refresh_mass(slot, slot.host.rec.data(), slot.host.n);
+    s->meshes.push_back(std::move(slot));
+  }
+  if (N >= (1ull << 32) || F >= (1ull << 30) || Sdir >= (1ull << 32) || Fent >= (1ull << 32))
+    return fail(FP_EINVAL, "simulation too large for 32-bit ids");
+  s->N = N;
+  s->F = F;
+
+  // state, masses, forces, pins
+  {
+    std::vector<float> comp(N), m(N);
+    for (int k = 0; k < 6; ++k) {
+      for (auto& ms_ : s->meshes)
+        for (uint64_t i = 0; i < ms_.host.n; ++i) comp[ms_.vbase + i] = ms_.host.rec[10 * i + 1 + k];
+      HIP_TRY(s->state[0][k].upload(comp.data(), N));
+      HIP_TRY(s->state[1][k].alloc(N));
+    }
+    for (auto& ms_ : s->meshes)
+      for (uint64_t i = 0; i < ms_.host.n; ++i) m[ms_.vbase + i] = ms_.host.rec[10 * i];
+    HIP_TRY(s->mass.upload(m.data(), N));
+    for (int k = 0; k < 3; ++k) {
+      for (auto& ms_ : s->meshes)
+        for (uint64_t i = 0; i < ms_.host.n; ++i) comp[ms_.vbase + i] = ms_.host.rec[10 * i + 7 + k];
+      HIP_TRY(s->force[k].upload(comp.data(), N));
+    }
+    bool nonzero_f = false;
+    for (auto& ms_ : s->meshes)
+      for (uint64_t i = 0; i < ms_.host.n && !nonzero_f; ++i)
+        nonzero_f = ms_.host.rec[10 * i + 7] != 0.0f || ms_.host.rec[10 * i + 8] != 0.0f || ms_.host.rec[10 * i + 9] != 0.0f ||
+                    std::signbit(ms_.host.rec[10 * i + 7]) || std::signbit(ms_.host.rec[10 * i + 8]) ||
+                    std::signbit(ms_.host.rec[10 * i + 9]);
+    s->forces_dirty = nonzero_f;
+    std::vector<uint32_t> bits((N + 31) / 32 + 1, 0u);
+    for (auto& ms_ : s->meshes)
+      for (uint64_t p : ms_.host.pins) {
+        uint64_t g = ms_.vbase + p;
+        bits[g >> 5] |= 1u << (g & 31);
+      }
+    HIP_TRY(s->pins.upload(bits.data(), bits.size()));
+  }
+
+  // CSR (vertex-centric, incident struts in strut order, incident faces in face order)
+  {
+    std::vector<uint32_t> srow(N + 1, 0), frow(N + 1, 0), fv(3 * F);
+    for (auto& ms_ : s->meshes) {
+      const auto& h = ms_.host;
+      for (uint64_t k = 0; k < h.n_struts(); ++k) {
+        srow[ms_.vbase + h.s_v0[k] + 1]++;
+        srow[ms_.vbase + h.s_v1[k] + 1]++;
+      }
+      for (uint64_t k = 0; k < h.f_v.size(); ++k) {
+        frow[ms_.vbase + h.f_v[k] + 1]++;
+        fv[3 * ms_.fbase + k] = (uint32_t)(ms_.vbase + h.f_v[k]);
+      }
+    }
+    for (uint64_t i = 0; i < N; ++i) { srow[i + 1] += srow[i]; frow[i + 1] += frow[i]; }
+    std::vector<uint32_t> snbr(srow[N]), fent(frow[N]);
+    std::vector<float> slen(srow[N]), sk(srow[N]), sc(srow[N]);
+    std::vector<uint32_t> sfill(srow.begin(), srow.end() - 1), ffill(frow.begin(), frow.end() - 1);
+    for (auto& ms_ : s->meshes) {
+      const auto& h = ms_.host;
+      for (uint64_t k = 0; k < h.n_struts(); ++k) {
+        uint32_t a = (uint32_t)(ms_.vbase + h.s_v0[k]), b = (uint32_t)(ms_.vbase + h.s_v1[k]);
+        uint32_t ea = sfill[a]++;
+        snbr[ea] = b; slen[ea] = h.s_len[k]; sk[ea] = h.s_k[k]; sc[ea] = h.s_c[k];
+        uint32_t eb = sfill[b]++;
+        snbr[eb] = a; slen[eb] = h.s_len[k]; sk[eb] = h.s_k[k]; sc[eb] = h.s_c[k];
+      }
+      for (uint64_t t = 0; t < h.n_faces(); ++t)
+        for (uint32_t j = 0; j < 3; ++j) {
+          uint32_t v = (uint32_t)(ms_.vbase + h.f_v[3 * t + j]);
+          fent[ffill[v]++] = (uint32_t)(((ms_.fbase + t) << 2) | j);
+        }
+    }
+    HIP_TRY(s->s_row.upload(srow.data(), srow.size()));
+    HIP_TRY(s->s_nbr.upload(snbr.data(), snbr.size()));
+    HIP_TRY(s->s_len.upload(slen.data(), slen.size()));
+    HIP_TRY(s->s_k.upload(sk.data(), sk.size()));
+    HIP_TRY(s->s_c.upload(sc.data(), sc.size()));
+    HIP_TRY(s->f_row.upload(frow.data(), frow.size()));
+    HIP_TRY(s->f_ent.upload(fent.data(), fent.size()));
+    HIP_TRY(s->f_v.upload(fv.data(), fv.size()));
+  }
+
+  // cloth descriptors, block table
+  {
+    std::vector<fp::ClothView> cv;
+    for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
+      MeshSlot& ms_ = s->meshes[mi];
+      const fp::ClothDesc& d = ms_.host.cloth;
+      fp::ClothView c{};
+      c.base = (uint32_t)ms_.vbase;
+      c.rows = d.rows;
+      c.cols = d.cols;
+      c.spacing = d.spacing;
+      c.cx = (float)d.cols / 2.0f * d.spacing;
+      c.cy = (float)d.rows / 2.0f * d.spacing;
+      c.p0 = d.position[0];
+      c.p1 = d.plane == FP_PLANE_XZ ? d.position[2] : d.position[1];
+      c.tk = d.tk; c.tc = d.tc; c.sk = d.sk; c.sc = d.sc; c.bk = d.bk; c.bc = d.bc;
+      cv.push_back(c);
+      ms_.blk_begin = (uint32_t)s->h_blocks.size();
+      for (uint64_t b = 0; b < ms_.host.n; b += fp::kBlock)
+        s->h_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.vbase + b),
+                                             (uint32_t)(ms_.vbase + std::min<uint64_t>(ms_.host.n, b + fp::kBlock)), 0u});
+      ms_.blk_end = (uint32_t)s->h_blocks.size();
+    }
+    HIP_TRY(s->cloths.upload(cv.data(), cv.size()));
+    HIP_TRY(s->all_blocks.upload(s->h_blocks.data(), s->h_blocks.size()));
+  }
+  std::vector<fp::HostObstacle> obs;
+  for (auto& o : os) obs.push_back(std::move(o->o));
+  if (int e = upload_obstacles(s.get(), obs)) return e;
+  if (int e = upload_params(s.get())) return e;
+  if (int e = build_block_lists(s.get())) return e;
+  HIP_TRY(hipDeviceSynchronize());
+  *out = s.release();
+  return FP_OK;
+}
+
+int fp_sim_set_config(fp_sim* s, const fp_config* c) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (int e = check_config(c)) return e;
+  for (auto& m : s->meshes) {
+    m.cfg = *c;
+    m.params = make_params(m.cfg, m.params.mass, m.uniform_mass);
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return upload_params(s);
+}
+
+int fp_sim_get_config(const fp_sim* s, fp_config* c) {
+  if (!s || !c) return fail(FP_EINVAL, "NULL argument");
+  *c = s->meshes.empty() ? default_config() : s->meshes[0].cfg;
+  return FP_OK;
+}
+
+int fp_sim_set_mesh_config(fp_sim* s, uint64_t mesh, const fp_config* c) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  if (int e = check_config(c)) return e;
+  MeshSlot& m = s->meshes[mesh];
+  m.cfg = *c;
+  m.params = make_params(m.cfg, m.params.mass, m.uniform_mass);
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return upload_params(s);
+}
+
+int fp_sim_step_async(fp_sim* s, uint64_t n) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  return run_steps(s, n);
+}
+
+int fp_sim_synchronize(fp_sim* s) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_sim_get_timestep(const fp_sim* s, float* out) {
+  if (!s || !out) return fail(FP_EINVAL, "NULL argument");
+  fp_config c = s->meshes.empty() ? default_config() : s->meshes[0].cfg;
+  if (!duration_roundtrip(c.dt, *out)) return fail(FP_EINVAL, "bad dt");
+  return FP_OK;
+}
+
+int fp_sim_step(fp_sim* s, uint64_t n, float* timestep_out) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (int e = run_steps(s, n)) return e;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (timestep_out) return fp_sim_get_timestep(s, timestep_out);
+  return FP_OK;
+}
+
+int fp_sim_mesh_count(const fp_sim* s, uint64_t* out) {
+  if (!s || !out) return fail(FP_EINVAL, "NULL argument");
+  *out = s->meshes.size();
+  return FP_OK;
+}
+
+int fp_sim_mesh_counts(const fp_sim* s, uint64_t mesh, uint64_t* np, uint64_t* ns, uint64_t* nf, uint64_t* npin) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  const fp::HostMesh& h = s->meshes[mesh].host;
+  if (np) *np = h.n;
+  if (ns) *ns = h.n_struts();
+  if (nf) *nf = h.n_faces();
+  if (npin) *npin = h.pins.size();
+  return FP_OK;
+}
+
+int fp_sim_get_points(fp_sim* s, uint64_t mesh, float* rec) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  if (!rec) return fail(FP_EINVAL, "records is NULL");
+  const MeshSlot& m = s->meshes[mesh];
+  const uint64_t n = m.host.n, b = m.vbase;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<float> comp(n);
+  auto get = [&](const float* d, int slot) -> int {
+    HIP_TRY(hipMemcpy(comp.data(), d + b, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (uint64_t i = 0; i < n; ++i) rec[10 * i + slot] = comp[i];
+    return FP_OK;
+  };
+  if (int e = get(s->mass.p, 0)) return e;
+  for (int k = 0; k < 6; ++k)
+    if (int e = get(s->state[s->cur][k].p, 1 + k)) return e;
+  for (int k = 0; k < 3; ++k)
+    if (int e = get(s->force[k].p, 7 + k)) return e;
+  return FP_OK;
+}
+
+int fp_sim_set_points(fp_sim* s, uint64_t mesh, const float* rec, uint64_t n) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  MeshSlot& m = s->meshes[mesh];
+  if (n != m.host.n) return fail(FP_EINVAL, "State Vector incorrect size! (record count != point count)");
+  if (n && !rec) return fail(FP_EINVAL, "records is NULL");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  std::vector<float> comp(n);
+  bool nonzero_f = false;
+  auto put = [&](float* d, int slot) -> int {
+    for (uint64_t i = 0; i < n; ++i) comp[i] = rec[10 * i + slot];
+    HIP_TRY(hipMemcpy(d + m.vbase, comp.data(), n * sizeof(float), hipMemcpyHostToDevice));
+    return FP_OK;
+  };
+  if (int e = put(s->mass.p, 0)) return e;
+  for (int k = 0; k < 6; ++k)
+    if (int e = put(s->state[s->cur][k].p, 1 + k)) return e;
+  for (int k = 0; k < 3; ++k) {
+    if (int e = put(s->force[k].p, 7 + k)) return e;
+    for (uint64_t i = 0; i < n && !nonzero_f; ++i) nonzero_f = rec[10 * i + 7 + k] != 0.0f || std::signbit(rec[10 * i + 7 + k]);
+  }
+  s->forces_dirty = s->forces_dirty || nonzero_f;
+  refresh_mass(m, rec, n);
+  if (int e = upload_params(s)) return e;
+  return build_block_lists(s);
+}
+
+int fp_sim_get_vertices(fp_sim* s, uint64_t mesh, float* pos, uint64_t* idx) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  const MeshSlot& m = s->meshes[mesh];
+  std::vector<float> rec(10 * m.host.n);
+  if (m.host.n)
+    if (int e = fp_sim_get_points(s, mesh, rec.data())) return e;
+  copy_vertices(m.host, rec.data(), pos, idx);
+  return FP_OK;
+}
+
+int fp_sim_accumulate_forces(fp_sim* s, uint64_t mesh) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  const MeshSlot& m = s->meshes[mesh];
+  fp::StepArgs a = s->args(true);
+  const fp::BlockRange* blk = s->all_blocks.p + m.blk_begin;
+  uint32_t nb = m.blk_end - m.blk_begin;
+  bool exact = s->math == FP_MATH_EXACT;
+  if (m.use_grid) HIP_TRY(fp::launch_grid(a, s->cloths.p, blk, nb, fp::MODE_FORCES, exact, s->stream));
+  else HIP_TRY(fp::launch_csr(a, s->csr(), blk, nb, fp::MODE_FORCES, exact, s->stream));
+  s->forces_dirty = true;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_sim_update_points(fp_sim* s, uint64_t mesh, const float* rec, uint64_t n) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  MeshSlot& m = s->meshes[mesh];
+  if (n != m.host.n) return fail(FP_EINVAL, "record count != point count");
+  if (n && !rec) return fail(FP_EINVAL, "records is NULL");
+  if (n == 0) return FP_OK;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(s->tmp_rec.upload(rec, 10 * n));
+  fp::StepArgs a = s->args(true);
+  HIP_TRY(fp::launch_update_points(a, s->tmp_rec.p, (uint32_t)m.vbase, s->mass.p, s->all_blocks.p + m.blk_begin,
+                                   m.blk_end - m.blk_begin, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->forces_dirty = true;
+  // masses may have changed: re-derive the uniform-mass flag from the device copy
+  std::vector<float> mm(n), r10(10 * n, 0.0f);
+  HIP_TRY(hipMemcpy(mm.data(), s->mass.p + m.vbase, n * sizeof(float), hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < n; ++i) r10[10 * i] = mm[i];
+  refresh_mass(m, r10.data(), n);
+  if (int e = upload_params(s)) return e;
+  return build_block_lists(s);
+}
+
+int fp_sim_clear_forces(fp_sim* s, uint64_t mesh) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  const MeshSlot& m = s->meshes[mesh];
+  for (int k = 0; k < 3; ++k) HIP_TRY(fp::launch_fill(s->force[k].p + m.vbase, m.host.n, 0.0f, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_sim_set_kernel(fp_sim* s, int32_t kernel, int32_t math) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (kernel < FP_KERNEL_AUTO || kernel > FP_KERNEL_GRID) return fail(FP_EINVAL, "unknown kernel");
+  if (math != FP_MATH_EXACT && math != FP_MATH_FAST) return fail(FP_EINVAL, "unknown math mode");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->kernel = kernel;
+  s->math = math;
+  return build_block_lists(s);
+}
+
+int fp_sim_time_steps(fp_sim* s, uint64_t n, float* ms) {
+  if (!s || !ms) return fail(FP_EINVAL, "NULL argument");
+  HIP_TRY(hipEventRecord(s->ev0, s->stream));
+  if (int e = run_steps(s, n)) return e;
+  HIP_TRY(hipEventRecord(s->ev1, s->stream));
+  HIP_TRY(hipEventSynchronize(s->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, s->ev0, s->ev1));
+  return FP_OK;
+}
+
+void fp_sim_free(fp_sim* s) { delete s; }
+
+}  // extern "C"

@@ -1,0 +1,67 @@
+// Vector arithmetic shared by the host builders and the gfx950 kernels.
+//
+// The exact policy reproduces the reference's f32 operation order (Rust never fuses a*b+c; cgmath
+// 0.18 defines dot as (x*x' + y*y') + z*z', normalize as v * (1/|v|), angle as atan2(|a x b|, a.b)).
+// All sources are compiled with -ffp-contract=off, so every a*b+c below stays two roundings unless
+// a policy asks for a fused multiply-add explicitly (the fast policy does).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define FP_HD __host__ __device__ __forceinline__
+
+namespace fp {
+
+struct f3 {
+  float x, y, z;
+};
+
+FP_HD f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+FP_HD f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+FP_HD f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+FP_HD f3 smul(float s, f3 v) { return f3{s * v.x, s * v.y, s * v.z}; }
+FP_HD f3 sdiv(f3 v, float s) { return f3{v.x / s, v.y / s, v.z / s}; }
+FP_HD f3 neg(f3 v) { return f3{-v.x, -v.y, -v.z}; }
+FP_HD f3 cross(f3 a, f3 b) { return f3{(a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x)}; }
+FP_HD float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+FP_HD bool sign_positive(float v) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return !__builtin_signbit(v);
+#else
+  return !signbit(v);
+#endif
+}
+
+// Host-side exact helpers (used by the topology builder for rest lengths and obstacle normals).
+inline float h_magnitude(f3 v) { return sqrtf(dot(v, v)); }
+inline f3 h_normalize(f3 v) { return smul(1.0f / h_magnitude(v), v); }
+
+// ---- math policies for the kernels ------------------------------------------------------------
+// ExactMath: correctly rounded sqrt and division (the HIP default for f32, left on), libm-grade
+// atan2, no fused multiply-add.  Matches the reference bit for bit except atan2 (glibc's atan2f
+// and the device's differ by about an ulp).
+struct ExactMath {
+  static constexpr bool kExact = true;
+  __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+  __device__ static float rcp(float x) { return 1.0f / x; }
+  __device__ static float div(float a, float b) { return a / b; }
+  __device__ static float atan2_(float y, float x) { return atan2f(y, x); }
+  __device__ static float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+  __device__ static float mad(float a, float b, float c) { return a * b + c; }
+};
+
+// FastMath: hardware sqrt/reciprocal (1 ulp), fused multiply-adds.  Not bit-identical; the parity
+// tests hold it to the 1e-5 relative one-step bound.
+struct FastMath {
+  static constexpr bool kExact = false;
+  __device__ static float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
+  __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+  __device__ static float atan2_(float y, float x) { return atan2f(y, x); }
+  __device__ static float dot(f3 a, f3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
+  __device__ static float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+};
+
+}  // namespace fp

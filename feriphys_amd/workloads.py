@@ -1,0 +1,211 @@
+"""Synthetic workloads for the springy step (numbers and choices are the documented benchmark ones).
+
+Each workload is described by plain arguments (constructor arguments of the reference API, a
+Config, obstacle triangles and a seeded initial-state jitter), so the same description can be fed
+to the GPU path (feriphys_amd.springy) and to the CPU oracle.
+
+Choices (the reference's literal constants diverge at dt = 1/600, so the cloth uses these):
+  * Spring constants are given per unit strain: the reference scales a nominal constant by
+    1/rest_length (springy_mesh.rs:83-84), so nominal = k * L_class makes the effective stiffness
+    k at the class's rest length (tensile L = s, shear L = s*sqrt(2), bend L = 2s).
+  * The reference adds Config.gravity as a force (springy_mesh.rs:194-196), so an acceleration g
+    is passed as the force m*g.
+  * The cloth lies in the xz plane (rows along +z) and "top" is the last row.
+  * Jitter: uniform +-1e-3 m on every coordinate of every non-pinned vertex, applied after
+    construction (rest lengths stay those of the exact grid), from numpy's PCG64 with the seed.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+F32 = np.float32
+
+
+@dataclass
+class ClothSpec:
+    rows: int
+    cols: int
+    spacing: float
+    position: Tuple[float, float, float]
+    point_mass: float
+    tk: float
+    tc: float
+    sk: float
+    sc: float
+    bk: float
+    bc: float
+    pins: List[int]
+    plane: int = 1  # xz
+
+    def args(self):
+        return (self.rows, self.cols, self.spacing, self.position, self.point_mass, self.tk, self.tc, self.sk,
+                self.sc, self.bk, self.bc, self.pins)
+
+
+@dataclass
+class ConfigSpec:
+    integration: int = 1
+    dt: float = 0.001
+    gravity: Tuple[float, float, float] = (0.0, -10.0, 0.0)
+    wind: Tuple[float, float, float] = (0.0, 0.0, 0.0)
+    lift_coefficient: float = 1.0
+    drag_coefficient: float = 1.0
+    coefficient_of_restitution: float = 0.95
+    coefficient_of_friction: float = 0.3
+
+    def f32(self) -> "ConfigSpec":
+        """All values rounded to f32 (the reference's storage)."""
+        f = lambda v: float(F32(v))
+        return ConfigSpec(self.integration, f(self.dt), tuple(f(x) for x in self.gravity),
+                          tuple(f(x) for x in self.wind), f(self.lift_coefficient), f(self.drag_coefficient),
+                          f(self.coefficient_of_restitution), f(self.coefficient_of_friction))
+
+
+@dataclass
+class Scene:
+    cloths: List[ClothSpec]
+    config: ConfigSpec
+    obstacles: List[Tuple[np.ndarray, np.ndarray]] = field(default_factory=list)  # (positions (n,3) f32, indices)
+    jitter: float = 1e-3
+    seed: int = 0
+    steps: int = 600
+    name: str = ""
+
+
+def per_strain(k: float, length: float) -> float:
+    """Nominal constant giving effective stiffness/damping k at rest length `length` (f32)."""
+    return float(F32(F32(k) * F32(length)))
+
+
+def cloth_spec(rows, cols, spacing=0.05, mass=0.01, tensile=(500.0, 2.0), shear=(300.0, 1.0), bend=(50.0, 0.5),
+               pins=(), position=(0.0, 0.0, 0.0), plane=1) -> ClothSpec:
+    s = float(F32(spacing))
+    ls, lsh, lb = s, float(F32(s * math.sqrt(2.0))), float(F32(2.0 * s))
+    return ClothSpec(rows, cols, s, tuple(float(F32(p)) for p in position), float(F32(mass)),
+                     per_strain(tensile[0], ls), per_strain(tensile[1], ls),
+                     per_strain(shear[0], lsh), per_strain(shear[1], lsh),
+                     per_strain(bend[0], lb), per_strain(bend[1], lb), [int(p) for p in pins], plane)
+
+
+def gravity_force(mass: float, g: float = 9.81):
+    return (0.0, float(F32(F32(-g) * F32(mass))), 0.0)
+
+
+def jitter_records(records: np.ndarray, pins: Sequence[int], amount: float, seed: int) -> np.ndarray:
+    """Adds seeded uniform +-amount to the positions of non-pinned vertices (columns 1..3)."""
+    rec = np.array(records, dtype=F32, copy=True)
+    rng = np.random.default_rng(seed)
+    d = rng.uniform(-amount, amount, size=(rec.shape[0], 3)).astype(F32)
+    if len(pins):
+        d[np.asarray(list(pins), dtype=np.int64)] = 0.0
+    rec[:, 1:4] = (rec[:, 1:4] + d).astype(F32)
+    return rec
+
+
+def floor_obstacle(y: float = -2.0, half: float = 1.0):
+    """The cloth demo's floor (demos/cloth.rs:250-259): a square at height y, normal +y."""
+    p = np.array([[-half, y, half], [half, y, half], [half, y, -half], [-half, y, -half]], F32)
+    return p, np.array([0, 1, 2, 0, 2, 3], np.uint64)
+
+
+def _rotation(ax, ay, az):
+    cx, sx, cy, sy, cz, sz = math.cos(ax), math.sin(ax), math.cos(ay), math.sin(ay), math.cos(az), math.sin(az)
+    rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return rz @ ry @ rx
+
+
+def icosphere(subdivisions: int):
+    """Unit icosphere: vertices (float64) and outward counter-clockwise triangles (int64)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+         [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]]
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+         [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5], [2, 4, 11],
+         [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    verts = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    for _ in range(subdivisions):
+        cache = {}
+        nf = []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[k] = len(verts) - 1
+            return cache[k]
+
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        f = nf
+    return np.array(verts), np.array(f, np.int64)
+
+
+# Rotation of the collider sphere chosen so that no face normal has a tiny component on the axis the
+# reference's containment test drops (that test picks the axis by SIGNED maximum,
+# collidable_mesh.rs:133-154); the smallest such component over the 320 faces is 0.0228.
+SPHERE_ROTATION = (5.34088472, 1.83004511, 1.68085871)
+
+
+def sphere_obstacle(radius: float, centre, subdivisions: int = 2, rotation=SPHERE_ROTATION):
+    v, f = icosphere(subdivisions)
+    v = v @ _rotation(*rotation).T
+    pos = (v * radius + np.asarray(centre, float)).astype(F32)
+    return pos, f.reshape(-1).astype(np.uint64)
+
+
+def config0(steps: int = 600, seed: int = 0, rows: int = 32, cols: int = 32, pins: str = "corners") -> Scene:
+    """One square cloth of 32x32 vertices, spacing 0.05 m, 0.01 kg per vertex; tensile (500, 2),
+    shear (300, 1), bend (50, 0.5); the two top corners pinned; gravity 9.81 m/s^2, wind (1, 0, 0.5)
+    m/s, drag 0.5, lift 0.2; RK4 at dt = 1/600 s; +-1e-3 m jitter.
+
+    pins="corners" is the specified case.  The reference's integrators amplify every undamped mode
+    (the frozen-force RK4 step has determinant 1 + h^2 K / 2), and the out-of-plane modes of a
+    tensioned cloth have no damping (dampers act along struts), so a cloth hanging from two corners
+    blows up after ~270 steps in the reference method itself.  pins="edge" pins the whole top row,
+    which keeps the tension low enough to stay bounded for 600 steps."""
+    pins = ([(rows - 1) * cols, rows * cols - 1] if pins == "corners"
+            else [(rows - 1) * cols + c for c in range(cols)])
+    c = cloth_spec(rows, cols, pins=pins)
+    cfg = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.01), (1.0, 0.0, 0.5), 0.2, 0.5, 0.95, 0.3).f32()
+    return Scene([c], cfg, [], 1e-3, seed, steps, f"cloth{rows}x{cols}")
+
+
+def config1(n: int = 1024, steps: int = 600, seed: int = 0, subdivisions: int = 2) -> Scene:
+    """One n x n grid cloth (1024: 1,048,576 vertices), config-0 strut classes; top edge pinned; one
+    icosphere collider of radius 0.3 x cloth width whose top is 0.5 m below the cloth (restitution
+    0.3, friction 0.2); RK4 at dt = 1/600 s; +-1e-3 m jitter."""
+    pins = [(n - 1) * n + c for c in range(n)]
+    cl = cloth_spec(n, n, pins=pins)
+    width = (n - 1) * cl.spacing
+    r = 0.3 * width
+    sphere = sphere_obstacle(r, (0.0, -(0.5 + r), 0.0), subdivisions)
+    cfg = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.01), (1.0, 0.0, 0.5), 0.2, 0.5, 0.3, 0.2).f32()
+    return Scene([cl], cfg, [sphere], 1e-3, seed, steps, f"cloth{n}x{n}+sphere")
+
+
+# ---- building a scene on either side ----------------------------------------------------------
+
+def build_gpu(scene: Scene):
+    """The scene as a feriphys_amd Simulation (records jittered, config set)."""
+    from . import springy as sp
+
+    meshes = []
+    for k, c in enumerate(scene.cloths):
+        cl = sp.Cloth(*c.args(), plane=c.plane)
+        if scene.jitter:
+            cl.mesh.set_points(jitter_records(cl.mesh.get_points(), c.pins, scene.jitter, scene.seed + k))
+        meshes.append(cl.mesh)
+    obs = [sp.CollidableMesh(p, i) for p, i in scene.obstacles]
+    sim = sp.Simulation(meshes, obs)
+    c = scene.config
+    sim.config = sp.Config(sp.Integration(c.integration), c.dt, c.gravity, c.wind, c.lift_coefficient,
+                           c.drag_coefficient, c.coefficient_of_restitution, c.coefficient_of_friction)
+    return sim
