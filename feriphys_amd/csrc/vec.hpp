@@ -12,6 +12,8 @@
 
 #define FP_HD __host__ __device__ __forceinline__
 
+#include "fdlibm_atan2f.hpp"
+
 namespace fp {
 
 struct f3 {
@@ -39,15 +41,15 @@ inline float h_magnitude(f3 v) { return sqrtf(dot(v, v)); }
 inline f3 h_normalize(f3 v) { return smul(1.0f / h_magnitude(v), v); }
 
 // ---- math policies for the kernels ------------------------------------------------------------
-// ExactMath: correctly rounded sqrt and division (the HIP default for f32, left on), libm-grade
-// atan2, no fused multiply-add.  Matches the reference bit for bit except atan2 (glibc's atan2f
-// and the device's differ by about an ulp).
+// ExactMath: correctly rounded sqrt and division (the HIP default for f32, left on), the C
+// library's atan2f rounding (fdlibm_atan2f.hpp), no fused multiply-add: the reference's f32
+// results bit for bit.
 struct ExactMath {
   static constexpr bool kExact = true;
   __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
   __device__ static float rcp(float x) { return 1.0f / x; }
   __device__ static float div(float a, float b) { return a / b; }
-  __device__ static float atan2_(float y, float x) { return atan2f(y, x); }
+  __device__ static float atan2_(float y, float x) { return fdlibm_atan2f(y, x); }
   __device__ static float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
   __device__ static float mad(float a, float b, float c) { return a * b + c; }
 };

@@ -1,9 +1,10 @@
 """GPU parity: the gfx950 step against the CPU oracle on the same inputs.
 
 Bars (written in each test): strut/face/pin index tables bit-exact (CPU tests); one-step forces
-within 1e-5 relative (to the largest force component) in f32; the EXACT kernels reproduce the
-oracle's states bit for bit wherever no atan2 is involved and to a few f32 ulps otherwise; 600-step
-trajectories within the stated position tolerance, with the f64 oracle reporting the error of both.
+within 1e-5 relative (to the largest force component) in f32 for every kernel; the EXACT kernels
+reproduce the oracle's forces and states bit for bit (their atan2 rounds like the C library's);
+600-step trajectories within the stated position tolerance, with the f64 oracle reporting the
+error of both.
 """
 import numpy as np
 import pytest
@@ -59,7 +60,7 @@ def test_one_step_forces_config0(name, kernel, math):
     assert np.isfinite(fg).all()
     assert err <= 1e-5
     if math == fa.FP_MATH_EXACT:
-        assert err <= 2e-6
+        assert (bits(fg) == bits(fr)).all()
 
 
 @pytest.mark.parametrize("name,kernel,math", KERNELS)
@@ -82,6 +83,8 @@ def test_one_step_state_config0(name, kernel, math):
     assert (sg[:, 7:10] == 0).all()               # forces cleared after the step
     assert dx <= 1e-5 * max(1.0, float(np.abs(sr[:, 1:4]).max()))
     assert dv <= 1e-5 * max(vscale, 1e-3)
+    if math == fa.FP_MATH_EXACT:
+        assert (bits(sg) == bits(sr)).all()
 
 
 def test_struts_only_bit_exact():
@@ -149,6 +152,8 @@ def test_trajectory_600_config0(name, kernel, math):
     assert e_gpu_32 <= TRAJ_TOL_M[math]
     # the GPU is no further from the f64 solution than a few times the f32 oracle is
     assert e_gpu_64 <= 4.0 * e_32_64 + 1e-6
+    if math == fa.FP_MATH_EXACT:
+        assert (bits(sg) == bits(s32)).all()
 
 
 @pytest.mark.parametrize("name,kernel,math", KERNELS[:2])
@@ -185,19 +190,44 @@ def _sphere_scene(n=24):
 
 
 @pytest.mark.parametrize("kernel", [fa.FP_KERNEL_GRID, fa.FP_KERNEL_CSR])
-def test_cloth_on_sphere_collision_trajectory(kernel):
-    """A cloth falling onto the icosphere collider: collisions every step from ~step 60 on."""
+def test_cloth_on_sphere_bit_exact_without_face_terms(kernel):
+    """A cloth falling onto the icosphere collider (collisions every step from ~step 60 on), with
+    drag and lift off so no atan2 is evaluated: bit-identical to the oracle for 240 steps."""
     sc = _sphere_scene()
     sc.steps = 240
+    sc.config.drag_coefficient = 0.0
+    sc.config.lift_coefficient = 0.0
+    sc.config.wind = (0.0, 0.0, 0.0)
     g = gpu_scene(sc, kernel, fa.FP_MATH_EXACT)
+    g.step(sc.steps)
+    o = oracle_scene(sc)
+    o.step(sc.steps)
+    assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
+
+
+@pytest.mark.parametrize("kernel,math", [(fa.FP_KERNEL_GRID, fa.FP_MATH_EXACT), (fa.FP_KERNEL_CSR, fa.FP_MATH_EXACT),
+                                         (fa.FP_KERNEL_GRID, fa.FP_MATH_FAST)])
+def test_cloth_on_sphere_collision_trajectory(kernel, math):
+    """The same with wind, drag and lift.  EXACT: bit-identical to the oracle.  FAST: face forces
+    differ by ulps, and a vertex that then meets the sphere one step earlier or later diverges by
+    up to the distance it travels in a step, so the bar is on the distribution: 99% of vertices
+    within 5e-3 m of the oracle and every vertex within 2e-2 m (one step of travel is ~1e-2 m)."""
+    sc = _sphere_scene()
+    sc.steps = 240
+    g = gpu_scene(sc, kernel, math)
     g.step(sc.steps)
     sg = g.get_points(0)
     o = oracle_scene(sc)
     o.step(sc.steps)
     sr = o.mesh(0).points()
-    e = float(np.abs(sg[:, 1:4].astype(np.float64) - sr[:, 1:4]).max())
-    announce(f"cloth on sphere, 240 steps, kernel {kernel}: max position error {e:.2e} m")
-    assert e <= 2e-4
+    e = np.abs(sg[:, 1:4].astype(np.float64) - sr[:, 1:4]).max(axis=1)
+    announce(f"cloth on sphere, 240 steps, kernel {kernel}: max {e.max():.2e} m, 99th pct "
+             f"{np.percentile(e, 99):.2e} m, median {np.median(e):.2e} m")
+    assert np.isfinite(sg).all()
+    if math == fa.FP_MATH_EXACT:
+        assert (bits(sg) == bits(sr)).all()
+    assert np.percentile(e, 99) <= 5e-3
+    assert e.max() <= 2e-2
 
 
 def test_ccd_first_hit_matches_oracle():
@@ -255,7 +285,7 @@ def test_config1_one_step_full_size():
     dx = float(np.abs(sg[:, 1:4].astype(np.float64) - sr[:, 1:4]).max())
     dv = float(np.abs(sg[:, 4:7].astype(np.float64) - sr[:, 4:7]).max())
     announce(f"config-1 one step: max |dx| {dx:.2e}, max |dv| {dv:.2e}")
-    assert dx <= 1e-6 and dv <= 1e-4
+    assert (bits(sg) == bits(sr)).all()
     c = gpu_scene(sc, fa.FP_KERNEL_CSR, fa.FP_MATH_EXACT)
     c.step(1)
     g.step(9)
