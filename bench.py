@@ -34,7 +34,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--n", type=int, default=1024, help="cloth side (vertices)")
     ap.add_argument("--math", choices=["exact", "fast"], default="exact")
-    ap.add_argument("--kernel", choices=["auto", "grid", "csr"], default="auto")
+    ap.add_argument("--kernel", choices=["auto", "grid", "grid_simple", "csr"], default="auto")
+    ap.add_argument("--tile", choices=["default", "16x64", "8x32", "16x16"], default="default")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     args = ap.parse_args()
 
@@ -49,8 +50,11 @@ def main():
 
     scene = wl.config1(n=args.n, steps=args.steps)
     sim = wl.build_gpu(scene)
-    kern = {"auto": fa.FP_KERNEL_AUTO, "grid": fa.FP_KERNEL_GRID, "csr": fa.FP_KERNEL_CSR}[args.kernel]
+    kern = {"auto": fa.FP_KERNEL_AUTO, "grid": fa.FP_KERNEL_GRID, "grid_simple": fa.FP_KERNEL_GRID_SIMPLE,
+            "csr": fa.FP_KERNEL_CSR}[args.kernel]
     sim.set_kernel(kern, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
+    if args.tile != "default":
+        sim.set_tile_shape(0, {"16x64": 0, "8x32": 1, "16x16": 2}[args.tile])
     N = scene.cloths[0].rows * scene.cloths[0].cols
     dt = scene.config.dt
 

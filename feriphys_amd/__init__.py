@@ -3,12 +3,14 @@
 The compute path is libferiphys_gpu.so (hand-written HIP kernels behind the C ABI of
 include/feriphys_gpu.h); this package is a thin ctypes mirror of the reference's Rust API.
 """
-from ._lib import (FP_EULER, FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_MATH_EXACT, FP_MATH_FAST,
+from ._lib import (FP_EULER, FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_KERNEL_GRID_SIMPLE, FP_MATH_EXACT,
+                   FP_MATH_FAST,
                    FP_PLANE_XY, FP_PLANE_XZ, FP_RK4, FpError, lib)
 from .springy import (Cloth, CollidableMesh, Config, Integration, Simulation, SpringyMesh, config_default)
 
 __all__ = [
     "Cloth", "CollidableMesh", "Config", "Integration", "Simulation", "SpringyMesh", "config_default", "FpError",
     "lib", "FP_EULER", "FP_RK4", "FP_PLANE_XY", "FP_PLANE_XZ", "FP_KERNEL_AUTO", "FP_KERNEL_CSR", "FP_KERNEL_GRID",
+    "FP_KERNEL_GRID_SIMPLE",
     "FP_MATH_EXACT", "FP_MATH_FAST",
 ]

@@ -13,8 +13,9 @@ LIB_PATH = os.path.join(_HERE, "libferiphys_gpu.so")
 FP_OK, FP_EINVAL, FP_ENONMANIFOLD, FP_EHIP, FP_ESTATE, FP_ENOMEM, FP_ENODEV = range(7)
 FP_EULER, FP_RK4 = 0, 1
 FP_PLANE_XY, FP_PLANE_XZ = 0, 1
-FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID = 0, 1, 2
+FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_KERNEL_GRID_SIMPLE = 0, 1, 2, 3
 FP_MATH_EXACT, FP_MATH_FAST = 0, 1
+FP_TILE_16x64, FP_TILE_8x32, FP_TILE_16x16 = 0, 1, 2
 
 # Every symbol declared in include/feriphys_gpu.h, with its ctypes signature.
 _U64 = C.c_uint64
@@ -80,6 +81,8 @@ SIGNATURES = {
     "fp_sim_update_points": (C.c_int, [_P, _U64, _FP, _U64]),
     "fp_sim_clear_forces": (C.c_int, [_P, _U64]),
     "fp_sim_set_kernel": (C.c_int, [_P, C.c_int32, C.c_int32]),
+    "fp_sim_set_tile_shape": (C.c_int, [_P, _U64, C.c_int32]),
+    "fp_sim_get_tile_shape": (C.c_int, [_P, _U64, C.POINTER(C.c_int32)]),
     "fp_sim_time_steps": (C.c_int, [_P, _U64, _FP]),
     "fp_sim_free": (None, [_P]),
 }

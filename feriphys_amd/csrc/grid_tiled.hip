@@ -1,0 +1,186 @@
+// LDS-tiled step kernel for meshes built by Cloth::new (springy/cloth.rs:20-113).
+//
+// One 256-thread workgroup owns a TR x TC tile of the grid:
+//   phase 0  stage x, y, z, vx, vy, vz of the tile plus a 2-vertex halo (bending struts reach two
+//            vertices) from HBM into LDS, coalesced along rows;
+//   phase 1  compute, once per tile, every face's drag + lift (each face is shared by three
+//            vertices) and the strut constants (rest length, stiffness, damping) per column pair,
+//            row pair and quad; a Cloth::new grid's rest lengths depend only on those;
+//   phase 2  every vertex gathers its struts (from LDS state and the constant tables) and its faces
+//            (from the phase-1 forces plus its own corner angle) in the reference's order,
+//            integrates, resolves collision and pins, and writes its new state to HBM.
+// The per-vertex operations and their order are those of grid_kernel, so the EXACT form returns
+// the same bits as the reference.  HBM traffic: the state read once (plus halo rows that the L2
+// serves) and written once.
+#include "kernels.hpp"
+
+namespace fp {
+
+namespace {
+
+// Strut constants (Strut::new, springy_mesh.rs:74-90) of a strut whose ends differ by dX, dY in the
+// construction grid: rest length |d| = sqrt(dX^2 + dY^2) (the third component is exactly 0),
+// stiffness and damping = nominal / length.
+struct StrutK {
+  float k, c, L;
+};
+__device__ __forceinline__ StrutK strut_k(float dX, float dY, float kn, float cn) {
+  const float L = __builtin_sqrtf(dX * dX + dY * dY);
+  const float inv = 1.0f / L;
+  return StrutK{inv * kn, inv * cn, L};
+}
+
+template <class M, int TR, int TC, int MODE>
+__global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const ClothView* __restrict__ cloths,
+                                                             const TileRange* __restrict__ tiles) {
+  constexpr int TH = TR + 4, TW = TC + 4;  // staged rows/cols: r0-2 .. r0+TR+1, c0-2 .. c0+TC+1
+  constexpr int QR = TR + 1, QC = TC + 1;  // quads (r', c') with r' in [r0-1, r0+TR), c' in [c0-1, c0+TC)
+  __shared__ float s_st[6][TH][TW];
+  __shared__ float s_face[2][QR][QC][3];
+  __shared__ StrutK s_shear[QR][QC];
+  __shared__ StrutK s_h[TC + 1], s_v[TR + 1], s_bh[TC + 2], s_bv[TR + 2];
+
+  const TileRange T = tiles[blockIdx.x];
+  const StepParams P = A.params[T.mesh];
+  const ClothView G = cloths[T.mesh];
+  const int R = (int)G.rows, C = (int)G.cols;
+  const int r0 = (int)T.r0, c0 = (int)T.c0;
+  const int tid = (int)threadIdx.x;
+  auto X = [&](int cc) -> float { return ((float)cc * G.spacing - G.cx) + G.p0; };
+  auto Y = [&](int rr) -> float { return ((float)rr * G.spacing - G.cy) + G.p1; };
+
+  // ---- phase 0: state tile + halo -> LDS
+  const float* src[6] = {A.in.x, A.in.y, A.in.z, A.in.vx, A.in.vy, A.in.vz};
+  for (int e = tid; e < TH * TW; e += kBlock) {
+    const int lr = e / TW, lc = e - lr * TW;
+    const int r = r0 - 2 + lr, c = c0 - 2 + lc;
+    if (r < 0 || r >= R || c < 0 || c >= C) continue;
+    const uint32_t g = G.base + (uint32_t)(r * C + c);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_st[q][lr][lc] = src[q][g];
+  }
+  // strut constant tables (construction grid only: no state needed)
+  for (int j = tid; j < TC + 1; j += kBlock) {  // (c0-1+j, c0+j)
+    const int c = c0 - 1 + j;
+    s_h[j] = strut_k(X(c) - X(c + 1), 0.0f, G.tk, G.tc);
+  }
+  for (int j = tid; j < TR + 1; j += kBlock) {  // rows (r0-1+j, r0+j)
+    const int r = r0 - 1 + j;
+    s_v[j] = strut_k(0.0f, Y(r) - Y(r + 1), G.tk, G.tc);
+  }
+  for (int j = tid; j < TC + 2; j += kBlock) {  // (c0-2+j, c0+j)
+    const int c = c0 - 2 + j;
+    s_bh[j] = strut_k(X(c) - X(c + 2), 0.0f, G.bk, G.bc);
+  }
+  for (int j = tid; j < TR + 2; j += kBlock) {  // rows (r0-2+j, r0+j)
+    const int r = r0 - 2 + j;
+    s_bv[j] = strut_k(0.0f, Y(r) - Y(r + 2), G.bk, G.bc);
+  }
+  for (int e = tid; e < QR * QC; e += kBlock) {  // anti-diagonal (r', c'+1)-(r'+1, c') of quad (r', c')
+    const int i = e / QC, j = e - i * QC;
+    const int r = r0 - 1 + i, c = c0 - 1 + j;
+    s_shear[i][j] = strut_k(X(c + 1) - X(c), Y(r) - Y(r + 1), G.sk, G.sc);
+  }
+  __syncthreads();
+
+  auto Pl = [&](int r, int c) { return f3{s_st[0][r - r0 + 2][c - c0 + 2], s_st[1][r - r0 + 2][c - c0 + 2],
+                                          s_st[2][r - r0 + 2][c - c0 + 2]}; };
+  auto Vl = [&](int r, int c) { return f3{s_st[3][r - r0 + 2][c - c0 + 2], s_st[4][r - r0 + 2][c - c0 + 2],
+                                          s_st[5][r - r0 + 2][c - c0 + 2]}; };
+
+  // ---- phase 1: drag + lift of every face that touches the tile, once per face
+  // A(q) = [(r',c'+1), (r'+1,c'), (r',c')], B(q) = [(r',c'+1), (r'+1,c'+1), (r'+1,c')]
+  for (int e = tid; e < QR * QC; e += kBlock) {
+    const int i = e / QC, j = e - i * QC;
+    const int r = r0 - 1 + i, c = c0 - 1 + j;
+    if (r < 0 || r > R - 2 || c < 0 || c > C - 2) continue;
+    const f3 pa = Pl(r, c + 1), pb = Pl(r + 1, c), pc = Pl(r, c), pd = Pl(r + 1, c + 1);
+    const f3 va = Vl(r, c + 1), vb = Vl(r + 1, c), vc = Vl(r, c), vd = Vl(r + 1, c + 1);
+    const f3 fa = face_force<M>(pa, pb, pc, va, vb, vc, P);
+    const f3 fb = face_force<M>(pa, pd, pb, va, vd, vb, P);
+    s_face[0][i][j][0] = fa.x; s_face[0][i][j][1] = fa.y; s_face[0][i][j][2] = fa.z;
+    s_face[1][i][j][0] = fb.x; s_face[1][i][j][1] = fb.y; s_face[1][i][j][2] = fb.z;
+  }
+  __syncthreads();
+
+  // ---- phase 2: per vertex
+  for (int e = tid; e < TR * TC; e += kBlock) {
+    const int lr = e / TC, lc = e - lr * TC;
+    const int r = r0 + lr, c = c0 + lc;
+    if (r >= R || c >= C) continue;
+    const uint32_t i = G.base + (uint32_t)(r * C + c);
+    const f3 p = Pl(r, c), v = Vl(r, c);
+    const float m = P.uniform_mass ? P.mass : A.mass[i];
+    f3 F = A.force_init ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
+    F = add(F, P.gravity);
+    auto strut = [&](int rr, int cc, const StrutK& K) {
+      strut_accumulate<M>(F, p, v, Pl(rr, cc), Vl(rr, cc), K.k, K.c, K.L);
+    };
+    const int qi = r - r0, qj = c - c0;  // quad (r-1, c-1) is s_*[qi][qj]
+    // struts in strut-index order (see grid_kernel in kernels.hip)
+    if (r >= 1 && c <= C - 2) { strut(r - 1, c + 1, s_shear[qi][qj + 1]); strut(r - 1, c, s_v[qi]); }
+    if (r <= R - 2 && c >= 1) { strut(r + 1, c - 1, s_shear[qi + 1][qj]); strut(r, c - 1, s_h[qj]); }
+    if (r <= R - 2 && c <= C - 2) { strut(r + 1, c, s_v[qi + 1]); strut(r, c + 1, s_h[qj + 1]); }
+    if (c == C - 1 && r >= 1) strut(r - 1, c, s_v[qi]);
+    if (r == R - 1 && c >= 1) strut(r, c - 1, s_h[qj]);
+    if (c == C - 1 && r <= R - 2) strut(r + 1, c, s_v[qi + 1]);
+    if (r == R - 1 && c <= C - 2) strut(r, c + 1, s_h[qj + 1]);
+    if (r >= 2 && c <= C - 3) strut(r - 2, c, s_bv[qi]);
+    if (c >= 2 && r <= R - 3) strut(r, c - 2, s_bh[qj]);
+    if (r <= R - 3 && c <= C - 3) { strut(r, c + 2, s_bh[qj + 2]); strut(r + 2, c, s_bv[qi + 2]); }
+    // faces in face-index order: A(r-1,c) A(r,c-1) A(r,c) B(r-1,c-1) B(r-1,c) B(r,c-1)
+    auto face = [&](int fs, int i2, int j2, float w) {
+      F = add(F, f3{w * s_face[fs][i2][j2][0], w * s_face[fs][i2][j2][1], w * s_face[fs][i2][j2][2]});
+    };
+    if (r >= 1 && c <= C - 2) face(0, qi, qj + 1, corner_weight<M>(p, Pl(r - 1, c), Pl(r - 1, c + 1)));
+    if (r <= R - 2 && c >= 1) face(0, qi + 1, qj, corner_weight<M>(p, Pl(r + 1, c - 1), Pl(r, c - 1)));
+    if (r <= R - 2 && c <= C - 2) face(0, qi + 1, qj + 1, corner_weight<M>(p, Pl(r, c + 1), Pl(r + 1, c)));
+    if (r >= 1 && c >= 1) face(1, qi, qj, corner_weight<M>(p, Pl(r, c - 1), Pl(r - 1, c)));
+    if (r >= 1 && c <= C - 2) face(1, qi, qj + 1, corner_weight<M>(p, Pl(r - 1, c + 1), Pl(r, c + 1)));
+    if (r <= R - 2 && c >= 1) face(1, qi + 1, qj, corner_weight<M>(p, Pl(r + 1, c), Pl(r + 1, c - 1)));
+    const bool pin = pinned(A.pins, i);
+    if (pin) F = f3{0.0f, 0.0f, 0.0f};
+    if (MODE == MODE_FORCES) {
+      A.fx[i] = F.x; A.fy[i] = F.y; A.fz[i] = F.z;
+      continue;
+    }
+    f3 xn, vn;
+    integrate<M>(P, m, p, v, F, xn, vn);
+    const int32_t hit = first_hit(A.obs, p, xn, P.dt_d);
+    if (hit >= 0) collide_response(A.obs.faces[hit], P, m, p, v, F, xn, vn);
+    if (pin) { xn = p; vn = v; }
+    A.out.x[i] = xn.x; A.out.y[i] = xn.y; A.out.z[i] = xn.z;
+    A.out.vx[i] = vn.x; A.out.vy[i] = vn.y; A.out.vz[i] = vn.z;
+    if (A.force_init) { A.fx[i] = 0.0f; A.fy[i] = 0.0f; A.fz[i] = 0.0f; }
+  }
+}
+
+template <class M, int TR, int TC>
+hipError_t launch_shape(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n, int mode,
+                        hipStream_t s) {
+  if (mode == MODE_STEP) grid_tiled_kernel<M, TR, TC, MODE_STEP><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
+  else grid_tiled_kernel<M, TR, TC, MODE_FORCES><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
+  return hipGetLastError();
+}
+
+template <class M>
+hipError_t launch_math(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n, int shape,
+                       int mode, hipStream_t s) {
+  switch (shape) {
+    case TILE_16x64: return launch_shape<M, 16, 64>(a, cloths, tiles, n, mode, s);
+    case TILE_8x32: return launch_shape<M, 8, 32>(a, cloths, tiles, n, mode, s);
+    case TILE_16x16: return launch_shape<M, 16, 16>(a, cloths, tiles, n, mode, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+hipError_t launch_grid_tiled(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n_tiles,
+                             int shape, int mode, bool exact, hipStream_t s) {
+  if (n_tiles == 0) return hipSuccess;
+  return exact ? launch_math<ExactMath>(a, cloths, tiles, n_tiles, shape, mode, s)
+               : launch_math<FastMath>(a, cloths, tiles, n_tiles, shape, mode, s);
+}
+
+}  // namespace fp

@@ -45,10 +45,24 @@ struct StepArgs {
 
 enum StepMode { MODE_STEP = 0, MODE_FORCES = 1 };
 
+// One workgroup's tile of a Cloth::new grid for the LDS-tiled kernel: rows [r0, r0+TR) and columns
+// [c0, c0+TC) of mesh `mesh` (clipped to the grid).
+struct TileRange {
+  uint32_t mesh, r0, c0, shape;
+};
+
+// Tile shapes compiled for the tiled grid kernel (rows x columns per 256-thread workgroup).
+enum TileShape { TILE_16x64 = 0, TILE_8x32 = 1, TILE_16x16 = 2, N_TILE_SHAPES = 3 };
+constexpr int kTileRows[N_TILE_SHAPES] = {16, 8, 16};
+constexpr int kTileCols[N_TILE_SHAPES] = {64, 32, 16};
+
 hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
                       bool exact, hipStream_t s);
 hipError_t launch_grid(const StepArgs& a, const ClothView* cloths, const BlockRange* blocks, uint32_t n_blocks,
                        int mode, bool exact, hipStream_t s);
+// LDS-tiled grid kernel over `n_tiles` tiles that all have tile shape `shape`.
+hipError_t launch_grid_tiled(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n_tiles,
+                             int shape, int mode, bool exact, hipStream_t s);
 // SpringyMesh::update_points on device: old state = a.in (+ force records, masses), new = `nw_rec`
 // (10-float Stateful records, indexed by global vertex id); the result is written in place into
 // a.in, the force records and the masses.

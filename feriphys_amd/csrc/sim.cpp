@@ -101,15 +101,26 @@ fp_config default_config() {
   return c;
 }
 
+enum MeshKernel { MK_CSR = 0, MK_GRID_SIMPLE = 1, MK_GRID_TILED = 2 };
+
 struct MeshSlot {
   fp::HostMesh host;
   uint64_t vbase = 0, fbase = 0;
   fp_config cfg;
   fp::StepParams params;
   bool uniform_mass = true;
-  bool use_grid = false;
+  int kernel = MK_CSR;
+  int tile_shape = fp::TILE_16x64;
   uint32_t blk_begin = 0, blk_end = 0;
+  uint32_t tile_begin = 0, tile_end = 0;  // range in fp_sim::mesh_tiles
 };
+
+// Default tile shape for a cloth of this size (measured on the 1024 x 1024 cloth: 16x16 and 8x32
+// tiles run at 7 workgroups per CU, 16x64 at 2; 8x32 wastes less on cloths under 16 rows).
+int default_tile_shape(uint32_t rows, uint32_t cols) {
+  if (rows < 16 && cols >= 32) return fp::TILE_8x32;
+  return fp::TILE_16x16;
+}
 
 }  // namespace
 
@@ -133,6 +144,10 @@ struct fp_sim {
   DBuf<fp::BlockRange> all_blocks, grid_blocks, csr_blocks;
   std::vector<fp::BlockRange> h_blocks;
   uint32_t n_grid_blocks = 0, n_csr_blocks = 0;
+  // LDS-tiled grid kernel: per mesh its tiles (all of the mesh's tile shape), and per shape the
+  // tiles of every tiled mesh in XCD-aware launch order
+  DBuf<fp::TileRange> tiles[fp::N_TILE_SHAPES], mesh_tiles;
+  uint32_t n_tiles[fp::N_TILE_SHAPES] = {0, 0, 0};
   DBuf<fp::ClothView> cloths;
   DBuf<fp::StepParams> params;
   int32_t kernel = FP_KERNEL_AUTO, math = FP_MATH_EXACT;
@@ -214,17 +229,57 @@ int upload_params(fp_sim* s) {
   return FP_OK;
 }
 
+// The tiles of one cloth in row-major tile order.
+void cloth_tiles(uint32_t mesh, const fp::ClothDesc& d, int shape, std::vector<fp::TileRange>& out) {
+  const uint32_t tr = (uint32_t)fp::kTileRows[shape], tc = (uint32_t)fp::kTileCols[shape];
+  for (uint32_t r = 0; r < d.rows; r += tr)
+    for (uint32_t c = 0; c < d.cols; c += tc) out.push_back(fp::TileRange{mesh, r, c, (uint32_t)shape});
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), each XCD with its
+// own L2.  Launch order is permuted so that XCD k receives the k-th contiguous eighth of the
+// tiles: vertically adjacent tiles, which read each other's halo rows, then share an L2.
+std::vector<fp::TileRange> xcd_order(const std::vector<fp::TileRange>& t) {
+  const size_t n = t.size(), per = (n + 7) / 8;
+  std::vector<fp::TileRange> out;
+  out.reserve(n);
+  for (size_t b = 0; b < 8 * per; ++b) {
+    const size_t logical = (b % 8) * per + b / 8;
+    if (logical < n) out.push_back(t[logical]);
+  }
+  return out;
+}
+
 int build_block_lists(fp_sim* s) {
   std::vector<fp::BlockRange> g, c;
-  for (auto& ms : s->meshes) {
-    bool grid = ms.host.cloth.valid && ms.uniform_mass && s->kernel != FP_KERNEL_CSR;
-    ms.use_grid = grid;
-    for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) (grid ? g : c).push_back(s->h_blocks[b]);
+  std::vector<fp::TileRange> by_shape[fp::N_TILE_SHAPES], all_tiles;
+  for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
+    MeshSlot& ms = s->meshes[mi];
+    const bool grid_ok = ms.host.cloth.valid && ms.uniform_mass;
+    ms.kernel = !grid_ok || s->kernel == FP_KERNEL_CSR ? MK_CSR
+                : s->kernel == FP_KERNEL_GRID_SIMPLE   ? MK_GRID_SIMPLE
+                                                       : MK_GRID_TILED;
+    ms.tile_begin = ms.tile_end = (uint32_t)all_tiles.size();
+    if (ms.kernel == MK_GRID_TILED) {
+      std::vector<fp::TileRange> mt;
+      cloth_tiles(mi, ms.host.cloth, ms.tile_shape, mt);
+      by_shape[ms.tile_shape].insert(by_shape[ms.tile_shape].end(), mt.begin(), mt.end());
+      all_tiles.insert(all_tiles.end(), mt.begin(), mt.end());
+      ms.tile_end = (uint32_t)all_tiles.size();
+      continue;
+    }
+    for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
   }
   s->n_grid_blocks = (uint32_t)g.size();
   s->n_csr_blocks = (uint32_t)c.size();
   HIP_TRY(s->grid_blocks.upload(g.data(), g.size()));
   HIP_TRY(s->csr_blocks.upload(c.data(), c.size()));
+  for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh) {
+    std::vector<fp::TileRange> o = xcd_order(by_shape[sh]);
+    s->n_tiles[sh] = (uint32_t)o.size();
+    HIP_TRY(s->tiles[sh].upload(o.data(), o.size()));
+  }
+  HIP_TRY(s->mesh_tiles.upload(all_tiles.data(), all_tiles.size()));
   return FP_OK;
 }
 
@@ -366,6 +421,9 @@ int run_steps(fp_sim* s, uint64_t n) {
   for (uint64_t k = 0; k < n; ++k) {
     fp::StepArgs a = s->args(s->forces_dirty);
     bool exact = s->math == FP_MATH_EXACT;
+    for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh)
+      HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->tiles[sh].p, s->n_tiles[sh], sh, fp::MODE_STEP, exact,
+                                    s->stream));
     HIP_TRY(fp::launch_grid(a, s->cloths.p, s->grid_blocks.p, s->n_grid_blocks, fp::MODE_STEP, exact, s->stream));
     HIP_TRY(fp::launch_csr(a, s->csr(), s->csr_blocks.p, s->n_csr_blocks, fp::MODE_STEP, exact, s->stream));
     s->cur ^= 1;
@@ -578,6 +636,7 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     Sdir += 2 * slot.host.n_struts();
     Fent += 3 * slot.host.n_faces();
     refresh_mass(slot, slot.host.rec.data(), slot.host.n);
+    if (slot.host.cloth.valid) slot.tile_shape = default_tile_shape(slot.host.cloth.rows, slot.host.cloth.cols);
     s->meshes.push_back(std::move(slot));
   }
   if (N >= (1ull << 32) || F >= (1ull << 30) || Sdir >= (1ull << 32) || Fent >= (1ull << 32))
@@ -829,7 +888,10 @@ int fp_sim_accumulate_forces(fp_sim* s, uint64_t mesh) {
   const fp::BlockRange* blk = s->all_blocks.p + m.blk_begin;
   uint32_t nb = m.blk_end - m.blk_begin;
   bool exact = s->math == FP_MATH_EXACT;
-  if (m.use_grid) HIP_TRY(fp::launch_grid(a, s->cloths.p, blk, nb, fp::MODE_FORCES, exact, s->stream));
+  if (m.kernel == MK_GRID_TILED)
+    HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->mesh_tiles.p + m.tile_begin, m.tile_end - m.tile_begin,
+                                  m.tile_shape, fp::MODE_FORCES, exact, s->stream));
+  else if (m.kernel == MK_GRID_SIMPLE) HIP_TRY(fp::launch_grid(a, s->cloths.p, blk, nb, fp::MODE_FORCES, exact, s->stream));
   else HIP_TRY(fp::launch_csr(a, s->csr(), blk, nb, fp::MODE_FORCES, exact, s->stream));
   s->forces_dirty = true;
   HIP_TRY(hipStreamSynchronize(s->stream));
@@ -868,12 +930,27 @@ int fp_sim_clear_forces(fp_sim* s, uint64_t mesh) {
 
 int fp_sim_set_kernel(fp_sim* s, int32_t kernel, int32_t math) {
   if (!s) return fail(FP_EINVAL, "sim is NULL");
-  if (kernel < FP_KERNEL_AUTO || kernel > FP_KERNEL_GRID) return fail(FP_EINVAL, "unknown kernel");
+  if (kernel < FP_KERNEL_AUTO || kernel > FP_KERNEL_GRID_SIMPLE) return fail(FP_EINVAL, "unknown kernel");
   if (math != FP_MATH_EXACT && math != FP_MATH_FAST) return fail(FP_EINVAL, "unknown math mode");
   HIP_TRY(hipStreamSynchronize(s->stream));
   s->kernel = kernel;
   s->math = math;
   return build_block_lists(s);
+}
+
+int fp_sim_set_tile_shape(fp_sim* s, uint64_t mesh, int32_t shape) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  if (shape < 0 || shape >= fp::N_TILE_SHAPES) return fail(FP_EINVAL, "unknown tile shape");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->meshes[mesh].tile_shape = shape;
+  return build_block_lists(s);
+}
+
+int fp_sim_get_tile_shape(const fp_sim* s, uint64_t mesh, int32_t* shape) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  if (!shape) return fail(FP_EINVAL, "shape is NULL");
+  *shape = s->meshes[mesh].tile_shape;
+  return FP_OK;
 }
 
 int fp_sim_time_steps(fp_sim* s, uint64_t n, float* ms) {

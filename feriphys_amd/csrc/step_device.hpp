@@ -94,9 +94,13 @@ __device__ __forceinline__ void strut_accumulate(f3& F, f3 p_self, f3 v_self, f3
 // (angle / PI) * this (:647-652).
 template <class M>
 __device__ __forceinline__ f3 face_force(f3 p0, f3 p1, f3 p2, f3 u0, f3 u1, f3 u2, const StepParams& P) {
-  f3 avg = f3{((u0.x + u1.x) + u2.x) / 3.0f, ((u0.y + u1.y) + u2.y) / 3.0f, ((u0.z + u1.z) + u2.z) / 3.0f};
-  if (!M::kExact) avg = f3{((u0.x + u1.x) + u2.x) * (1.0f / 3.0f), ((u0.y + u1.y) + u2.y) * (1.0f / 3.0f),
-                           ((u0.z + u1.z) + u2.z) * (1.0f / 3.0f)};
+  // (v0 + v1 + v2) / 3.0 (springy_mesh.rs:634); div3_rn is that division, correctly rounded
+  f3 avg;
+  if constexpr (M::kExact)
+    avg = f3{div3_rn((u0.x + u1.x) + u2.x), div3_rn((u0.y + u1.y) + u2.y), div3_rn((u0.z + u1.z) + u2.z)};
+  else
+    avg = f3{((u0.x + u1.x) + u2.x) * (1.0f / 3.0f), ((u0.y + u1.y) + u2.y) * (1.0f / 3.0f),
+             ((u0.z + u1.z) + u2.z) * (1.0f / 3.0f)};
   f3 rv = sub(avg, P.wind);
   f3 cr = cross(sub(p1, p0), sub(p2, p0));
   float cm = M::sqrt_(M::dot(cr, cr));
@@ -122,7 +126,7 @@ __device__ __forceinline__ float corner_weight(f3 pa, f3 pnext, f3 pprev) {
   f3 b = sub(pnext, pa);
   f3 axb = cross(a, b);
   float th = M::atan2_(M::sqrt_(M::dot(axb, axb)), M::dot(a, b));
-  return th / 3.14159274101257324f;  // Rad / Rad(PI), PI = std::f32::consts::PI
+  return M::div(th, 3.14159274101257324f);  // Rad / Rad(PI), PI = std::f32::consts::PI
 }
 
 // The explicit integrators of state.rs:75-106 applied to one Point record (springy_mesh.rs:223-240)
@@ -137,9 +141,9 @@ __device__ __forceinline__ void integrate(const StepParams& P, float m, f3 x, f3
     return;
   }
   // Stages 2-4 see the force record F + 0 (the force slot's derivative is 0), which differs from F
-  // only in the sign of a zero.
-  f3 F0 = f3{F.x + 0.0f, F.y + 0.0f, F.z + 0.0f};
-  f3 a2 = f3{M::div(F0.x, m), M::div(F0.y, m), M::div(F0.z, m)};
+  // only for F = -0: then (F + 0) / m = +0 / m, otherwise it is a1.
+  const float z0 = M::div(0.0f, m);
+  f3 a2 = f3{F.x == 0.0f ? z0 : a1.x, F.y == 0.0f ? z0 : a1.y, F.z == 0.0f ? z0 : a1.z};
   const float hh = P.dt_half, h = P.dt, h6 = P.dt6, h3 = P.dt3;
   f3 v2 = f3{v.x + a1.x * hh, v.y + a1.y * hh, v.z + a1.z * hh};
   f3 v3 = f3{v.x + a2.x * hh, v.y + a2.y * hh, v.z + a2.z * hh};
@@ -155,12 +159,14 @@ __device__ __forceinline__ void integrate(const StepParams& P, float m, f3 x, f3
 
 // CollidableMesh::get_collided_face_from_list's per-face predicate (collidable_mesh.rs:113-163).
 // Always exact: it decides a branch, and its inputs are the same bits in every math policy.
-__device__ __forceinline__ bool face_hit(const ObsFace& f, f3 old_p, f3 new_p, f3 ov, float dt_d) {
+__device__ __forceinline__ bool face_hit(const ObsFace& f, f3 old_p, f3 new_p, float dt_d) {
   f3 n = f3{f.n[0], f.n[1], f.n[2]};
   f3 v0 = f3{f.v0[0], f.v0[1], f.v0[2]};
   float od = dot(sub(old_p, v0), n);
   float nd = dot(sub(new_p, v0), n);
   if (sign_positive(od) == sign_positive(nd)) return false;
+  // old_velocity = (new - old) / dt (collidable_mesh.rs:114), needed only once a plane is crossed
+  const f3 ov = f3{(new_p.x - old_p.x) / dt_d, (new_p.y - old_p.y) / dt_d, (new_p.z - old_p.z) / dt_d};
   float frac = od / (od - nd);
   float s = dt_d * frac;
   f3 cp = f3{old_p.x + s * ov.x, old_p.y + s * ov.y, old_p.z + s * ov.z};
@@ -177,7 +183,7 @@ __device__ __forceinline__ bool face_hit(const ObsFace& f, f3 old_p, f3 new_p, f
 // Lowest-index face hit in one grid by the segment whose grown box is [lo, hi]; `best` is the
 // lowest hit so far (faces at or above it need no test).
 __device__ __forceinline__ void grid_hits(const GridView& G, const ObsFace* faces, const float lo[3], const float hi[3],
-                                          f3 old_p, f3 new_p, f3 ov, float dt_d, uint32_t& best) {
+                                          f3 old_p, f3 new_p, float dt_d, uint32_t& best) {
   if (G.cell_start == nullptr) return;
   int32_t c0[3], c1[3];
   for (int a = 0; a < 3; ++a) {
@@ -194,7 +200,7 @@ __device__ __forceinline__ void grid_hits(const GridView& G, const ObsFace* face
         for (uint32_t k = G.cell_start[cell], e = G.cell_start[cell + 1]; k < e; ++k) {
           const uint32_t f = G.cell_faces[k];
           if (f >= best) break;
-          if (face_hit(faces[f], old_p, new_p, ov, dt_d)) { best = f; break; }
+          if (face_hit(faces[f], old_p, new_p, dt_d)) { best = f; break; }
         }
       }
 }
@@ -204,24 +210,22 @@ __device__ __forceinline__ void grid_hits(const GridView& G, const ObsFace* face
 // candidates are fewer but the answer is the same: the lowest index whose predicate holds.
 __device__ __forceinline__ int32_t first_hit(const ObsView& O, f3 old_p, f3 new_p, float dt_d) {
   if (O.n_faces == 0) return -1;
-  // old_velocity = (new - old) / dt (collidable_mesh.rs:114)
-  const f3 ov = f3{(new_p.x - old_p.x) / dt_d, (new_p.y - old_p.y) / dt_d, (new_p.z - old_p.z) / dt_d};
   if (!O.use_grid) {
     for (uint32_t f = 0; f < O.n_faces; ++f)
-      if (face_hit(O.faces[f], old_p, new_p, ov, dt_d)) return (int32_t)f;
+      if (face_hit(O.faces[f], old_p, new_p, dt_d)) return (int32_t)f;
     return -1;
   }
   uint32_t best = 0xffffffffu;
   for (uint32_t k = 0; k < O.n_always; ++k) {
     const uint32_t f = O.always[k];
     if (f >= best) break;
-    if (face_hit(O.faces[f], old_p, new_p, ov, dt_d)) { best = f; break; }
+    if (face_hit(O.faces[f], old_p, new_p, dt_d)) { best = f; break; }
   }
   const float lo[3] = {fminf(old_p.x, new_p.x) - O.margin, fminf(old_p.y, new_p.y) - O.margin,
                        fminf(old_p.z, new_p.z) - O.margin};
   const float hi[3] = {fmaxf(old_p.x, new_p.x) + O.margin, fmaxf(old_p.y, new_p.y) + O.margin,
                        fmaxf(old_p.z, new_p.z) + O.margin};
-  for (int g = 0; g < 4; ++g) grid_hits(O.grid[g], O.faces, lo, hi, old_p, new_p, ov, dt_d, best);
+  for (int g = 0; g < 4; ++g) grid_hits(O.grid[g], O.faces, lo, hi, old_p, new_p, dt_d, best);
   return best == 0xffffffffu ? -1 : (int32_t)best;
 }
 

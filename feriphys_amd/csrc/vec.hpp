@@ -12,6 +12,7 @@
 
 #define FP_HD __host__ __device__ __forceinline__
 
+#include "exact_ops.hpp"
 #include "fdlibm_atan2f.hpp"
 
 namespace fp {
@@ -54,14 +55,35 @@ struct ExactMath {
   __device__ static float mad(float a, float b, float c) { return a * b + c; }
 };
 
-// FastMath: hardware sqrt/reciprocal (1 ulp), fused multiply-adds.  Not bit-identical; the parity
-// tests hold it to the 1e-5 relative one-step bound.
+// atan2 for the fast kernels: atan of min/max(|x|,|y|) in [0, 1] by an odd degree-13 polynomial
+// (least-squares fit on Chebyshev nodes, refined towards minimax; max error 3.2e-7 rad), then
+// placed in its octant.
+__device__ __forceinline__ float fast_atan2(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float a = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+  const float t = a * a;
+  float p = 0.006811788771301508f;
+  p = __builtin_fmaf(p, t, -0.0336042158305645f);
+  p = __builtin_fmaf(p, t, 0.07962367683649063f);
+  p = __builtin_fmaf(p, t, -0.132333442568779f);
+  p = __builtin_fmaf(p, t, 0.19807817041873932f);
+  p = __builtin_fmaf(p, t, -0.3331736922264099f);
+  p = __builtin_fmaf(p, t, 0.9999961256980896f);
+  float r = p * a;
+  if (ay > ax) r = 1.5707963267948966f - r;
+  if (x < 0.0f) r = 3.141592653589793f - r;
+  return __builtin_copysignf(r, y);
+}
+
+// FastMath: hardware sqrt/reciprocal (1 ulp), fused multiply-adds, polynomial atan2.  Not
+// bit-identical; the parity tests hold it to the 1e-5 relative one-step bound.
 struct FastMath {
   static constexpr bool kExact = false;
   __device__ static float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
   __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
-  __device__ static float atan2_(float y, float x) { return atan2f(y, x); }
+  __device__ static float atan2_(float y, float x) { return fast_atan2(y, x); }
   __device__ static float dot(f3 a, f3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
   __device__ static float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };

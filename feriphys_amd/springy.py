@@ -298,6 +298,14 @@ class Simulation(_Handle):
     def set_kernel(self, kernel: int = _lib.FP_KERNEL_AUTO, math: int = _lib.FP_MATH_EXACT):
         check(lib().fp_sim_set_kernel(self.handle, kernel, math))
 
+    def set_tile_shape(self, mesh: int, shape: int):
+        check(lib().fp_sim_set_tile_shape(self.handle, mesh, shape))
+
+    def get_tile_shape(self, mesh: int = 0) -> int:
+        v = C.c_int32()
+        check(lib().fp_sim_get_tile_shape(self.handle, mesh, C.byref(v)))
+        return v.value
+
     def time_steps(self, n_steps: int) -> float:
         """Runs n_steps bracketed by HIP events on the simulation's stream; returns milliseconds."""
         ms = C.c_float()

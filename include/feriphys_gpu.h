@@ -175,16 +175,27 @@ int fp_sim_update_points(fp_sim* sim, uint64_t mesh, const float* new_records, u
 /* SpringyMesh::clear_forces (springy_mesh.rs:656-660). */
 int fp_sim_clear_forces(fp_sim* sim, uint64_t mesh);
 
-/* Kernel selection.  FP_KERNEL_AUTO uses the grid kernel for meshes built by Cloth::new and left
- * unchanged, the CSR kernel otherwise.  EXACT kernels follow the reference's operation order with
- * no FMA contraction and correctly rounded division and square root; FAST kernels let the
- * compiler contract and use the hardware reciprocal/rsqrt (results within 1e-5 relative). */
+/* Kernel selection.  FP_KERNEL_AUTO uses the LDS-tiled grid kernel for meshes built by Cloth::new
+ * and left unchanged, the CSR kernel otherwise; FP_KERNEL_GRID_SIMPLE is the one-thread-per-vertex
+ * grid kernel without LDS (kept as a cross-check).  EXACT kernels follow the reference's operation
+ * order with no FMA contraction, correctly rounded division and square root and the C library's
+ * atan2f rounding (bit-identical results); FAST kernels use fused multiply-adds and the hardware
+ * reciprocal and square root (one-step forces within 1e-5 relative). */
 #define FP_KERNEL_AUTO 0
 #define FP_KERNEL_CSR 1
 #define FP_KERNEL_GRID 2
+#define FP_KERNEL_GRID_SIMPLE 3
 #define FP_MATH_EXACT 0
 #define FP_MATH_FAST 1
 int fp_sim_set_kernel(fp_sim* sim, int32_t kernel, int32_t math);
+/* Tile shape of the LDS-tiled grid kernel for one cloth mesh (rows x columns per workgroup):
+ * FP_TILE_16x64, FP_TILE_8x32 or FP_TILE_16x16.  The default is chosen from the cloth's size;
+ * fp_sim_autotune times the shapes and keeps the fastest per mesh-size class. */
+#define FP_TILE_16x64 0
+#define FP_TILE_8x32 1
+#define FP_TILE_16x16 2
+int fp_sim_set_tile_shape(fp_sim* sim, uint64_t mesh, int32_t shape);
+int fp_sim_get_tile_shape(const fp_sim* sim, uint64_t mesh, int32_t* shape);
 /* Runs n_steps on the simulation's stream bracketed by HIP events recorded on that stream;
  * kernel_ms receives the event-measured time of the whole run (device time, inputs resident). */
 int fp_sim_time_steps(fp_sim* sim, uint64_t n_steps, float* kernel_ms);

@@ -15,3 +15,13 @@ def test_fdlibm_atan2f_matches_libm(tmp_path):
                     os.path.join(HERE, "native", "atan2f_check.cpp"), "-o", exe, "-lm"], check=True)
     out = subprocess.run([exe, "20000000"], check=True, capture_output=True, text=True).stdout.strip().splitlines()
     assert out[-1] == "0", "\n".join(out[-20:])
+
+
+def test_div3_rn_exhaustive(tmp_path):
+    """div3_rn (the exact kernels' x / 3.0f without a division) equals x / 3.0f on all 2^32 - 2^24
+    finite f32 inputs."""
+    exe = str(tmp_path / "div3_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-mfma", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+                    os.path.join(HERE, "native", "div3_check.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.strip()
+    assert out == "0"

@@ -117,15 +117,21 @@ def test_euler_struts_only_bit_exact():
     assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
 
 
-def test_grid_and_csr_kernels_identical():
-    """The grid kernel rebuilds struts, rest lengths and faces from (row, col); the CSR kernel reads
-    the host-built tables.  Same summation order, so the EXACT forms agree bit for bit."""
-    sc = wl.config0(steps=200)
-    a = gpu_scene(sc, fa.FP_KERNEL_GRID, fa.FP_MATH_EXACT)
-    b = gpu_scene(sc, fa.FP_KERNEL_CSR, fa.FP_MATH_EXACT)
-    a.step(sc.steps)
-    b.step(sc.steps)
-    assert (bits(a.get_points(0)) == bits(b.get_points(0))).all()
+@pytest.mark.parametrize("rows,cols", [(32, 32), (37, 70), (5, 3), (2, 2), (19, 129)])
+def test_grid_and_csr_kernels_identical(rows, cols):
+    """The grid kernels rebuild struts, rest lengths and faces from (row, col) (the tiled one via
+    LDS tables and once-per-face forces); the CSR kernel reads the host-built tables.  Same
+    summation order, so the EXACT forms agree bit for bit, also for grids that do not fill their
+    last tiles."""
+    sc = wl.config0(steps=100, rows=rows, cols=cols, pins="edge")
+    outs = []
+    for k in (fa.FP_KERNEL_GRID, fa.FP_KERNEL_GRID_SIMPLE, fa.FP_KERNEL_CSR):
+        g = gpu_scene(sc, k, fa.FP_MATH_EXACT)
+        g.step(sc.steps)
+        outs.append(g.get_points(0))
+    assert np.isfinite(outs[0]).all()
+    assert (bits(outs[0]) == bits(outs[2])).all()
+    assert (bits(outs[1]) == bits(outs[2])).all()
 
 
 @pytest.mark.parametrize("name,kernel,math", KERNELS)
@@ -291,3 +297,8 @@ def test_config1_one_step_full_size():
     g.step(9)
     c.step(9)
     assert (bits(g.get_points(0)) == bits(c.get_points(0))).all()
+    f = gpu_scene(sc, fa.FP_KERNEL_GRID, fa.FP_MATH_FAST)
+    f.step(10)
+    e = float(np.abs(f.get_points(0)[:, 1:4].astype(np.float64) - g.get_points(0)[:, 1:4]).max())
+    announce(f"config-1 10 steps fast vs exact: max position difference {e:.2e} m")
+    assert e <= 1e-5
