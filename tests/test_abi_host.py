@@ -1,0 +1,186 @@
+"""CPU tests of the C ABI library: it loads, exports every declared symbol, and its host-side
+constructors (no GPU needed) build the same tables as the oracle, bit for bit."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import feriphys_amd as fa
+import pyoracle as orc
+from feriphys_amd import _lib
+from feriphys_amd import workloads as wl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "feriphys_gpu.h")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fp_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = fa.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 40
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r"\bT (fp_[a-z0-9_]+)", out))
+    missing = [s for s in syms if s not in exported]
+    assert not missing, missing
+    for s in syms:
+        getattr(lib, s)
+    # the Python mirror binds exactly the declared entry points
+    assert set(_lib.SIGNATURES) == set(syms)
+
+
+def test_config_default():
+    # springy/config.rs:20-33
+    c = fa.config_default()
+    assert c.integration == fa.Integration.Rk4
+    assert c.dt == np.float32(0.001)
+    assert c.gravity == (0.0, -10.0, 0.0) and c.wind == (0.0, 0.0, 0.0)
+    assert (c.lift_coefficient, c.drag_coefficient) == (1.0, 1.0)
+    assert c.coefficient_of_restitution == np.float32(0.95) and c.coefficient_of_friction == np.float32(0.3)
+
+
+def compare_mesh(g, o):
+    assert g.counts() == o.counts()
+    v01, kcl, fc = g.struts()
+    so = o.struts()
+    assert (v01 == so["v01"]).all()
+    assert (bits(kcl) == bits(so["kcl"])).all()
+    assert (fc == so["faces"]).all()
+    s3, v3 = g.faces()
+    os3, ov3 = o.faces()
+    assert (s3 == os3).all() and (v3 == ov3).all()
+    assert (bits(g.get_points()) == bits(o.points())).all()
+    assert (g.pins() == o.pins()).all()
+    has, kcr = g.torsion()
+    assert (has == so["has_torsion"]).all()
+    assert (bits(kcr[has]) == bits(so["torsion"][has])).all()
+
+
+@pytest.mark.parametrize("rows,cols,plane", [(32, 32, 1), (32, 32, 0), (5, 7, 0), (2, 2, 0), (3, 9, 1), (2, 11, 1),
+                                             (17, 3, 0), (64, 48, 1)])
+def test_cloth_tables_match_oracle(rows, cols, plane):
+    c = wl.config0(rows=rows, cols=cols).cloths[0]
+    compare_mesh(fa.Cloth(*c.args(), plane=plane).mesh, orc.Mesh.cloth(*c.args(), plane=plane))
+
+
+def test_cloth_1024_tables_match_oracle():
+    """The benchmark cloth: 1,048,576 vertices, 5,230,601 struts (one shear diagonal per quad,
+    truncated bends), 2,093,058 faces."""
+    c = wl.config1().cloths[0]
+    g = fa.Cloth(*c.args(), plane=c.plane).mesh
+    o = orc.Mesh.cloth(*c.args(), plane=c.plane)
+    assert g.counts() == (1048576, 5230601, 2093058, 1024)
+    compare_mesh(g, o)
+
+
+@pytest.mark.parametrize("rows,cols", [(2, 2), (3, 3), (4, 7), (10, 10), (31, 5)])
+def test_cloth_counts_closed_form(rows, cols):
+    """S = 3(R-1)(C-1) + (R-1) + (C-1) + 2(R-2)(C-2); F = 2(R-1)(C-1) (springy/cloth.rs:63-110)."""
+    c = wl.config0(rows=rows, cols=cols).cloths[0]
+    n, s, f, p = fa.Cloth(*c.args()).mesh.counts()
+    assert n == rows * cols
+    assert s == 3 * (rows - 1) * (cols - 1) + (rows - 1) + (cols - 1) + 2 * max(rows - 2, 0) * max(cols - 2, 0)
+    assert f == 2 * (rows - 1) * (cols - 1)
+
+
+def test_three_by_three_cloth_strut_list_by_hand():
+    """Brute force on a tiny input: every strut of the 3x3 cloth, enumerated by hand from the
+    triangle list (top-left triangles (i+1, i+3, i) of quads 0, 1, 3, 4, then bottom-right
+    triangles (i-2, i+1, i) for i = 3, 4, 6, 7, which add only the last column's verticals and the
+    last row's horizontals) and the two bending struts (0,2), (0,6)."""
+    c = wl.config0(rows=3, cols=3).cloths[0]
+    v01, kcl, _ = fa.Cloth(*c.args()).mesh.struts()
+    expect = [(1, 3), (0, 3), (0, 1), (2, 4), (1, 4), (1, 2), (4, 6), (3, 6), (3, 4), (5, 7), (4, 7), (4, 5),
+              (2, 5), (6, 7), (5, 8), (7, 8), (0, 2), (0, 6)]
+    assert [tuple(map(int, r)) for r in v01] == expect
+    s = np.float32(c.spacing)
+    L = kcl[:, 2]
+    assert abs(L[2] - s) < 1e-7 and abs(L[1] - s) < 1e-7         # tensile: the spacing
+    assert abs(L[0] - s * np.sqrt(2)) < 1e-7                      # shear diagonal
+    assert abs(L[16] - 2 * s) < 1e-7 and abs(L[17] - 2 * s) < 1e-7  # bends
+    # shear struts carry the shear override, the rest the tensile/bend constants (1/L * nominal)
+    assert kcl[0, 0] == np.float32(1.0) / L[0] * np.float32(c.sk)
+    assert kcl[2, 0] == np.float32(1.0) / L[2] * np.float32(c.tk)
+    assert kcl[13, 0] == np.float32(1.0) / L[13] * np.float32(c.tk)
+    assert kcl[16, 0] == np.float32(1.0) / L[16] * np.float32(c.bk)
+
+
+def test_random_mesh_tables_match_oracle():
+    rng = np.random.default_rng(3)
+    for trial in range(5):
+        n = 40
+        pos = rng.normal(size=(n, 3)).astype(np.float32)
+        # a random manifold-ish strip: fan triangles with distinct corners
+        tris = []
+        for t in range(60):
+            a, b, c = rng.choice(n, 3, replace=False)
+            tris.append((a, b, c))
+        try:
+            o = orc.Mesh.new(pos, np.array(tris).reshape(-1), 2.5, 7.0, 0.5, torsion=(3.0, 1.0),
+                             overrides=[(1, 2, 9.0, 9.5), (int(tris[0][0]), int(tris[0][1]), 4.0, 4.5)])
+        except ValueError:
+            with pytest.raises(ValueError):
+                fa.SpringyMesh(pos, np.array(tris).reshape(-1), 2.5, 7.0, 0.5, (3.0, 1.0),
+                               {(1, 2): (9.0, 9.5), (int(tris[0][0]), int(tris[0][1])): (4.0, 4.5)})
+            continue
+        g = fa.SpringyMesh(pos, np.array(tris).reshape(-1), 2.5, 7.0, 0.5, (3.0, 1.0),
+                           {(1, 2): (9.0, 9.5), (int(tris[0][0]), int(tris[0][1])): (4.0, 4.5)})
+        for k in range(5):
+            a, b = rng.choice(n, 2)
+            g.add_strut((a, b), 1.5 + k, 0.25)
+            o.add_strut(int(a), int(b), 1.5 + k, 0.25)
+            g.add_pin(int(a))
+            o.add_pin(int(a))
+        compare_mesh(g, o)
+
+
+def test_strip_and_triangle_like_reference_tests():
+    # springy_mesh.rs:696-719 and :773-866 through the product library
+    pos = [[0, 0, 0], [1, 0, 0], [1, 0, 1], [-1, 1, 0], [0, 1, 0], [1, 1, 0]]
+    idx = [0, 4, 3, 0, 1, 4, 1, 5, 4, 1, 2, 5]
+    g = fa.SpringyMesh(pos, idx, 1.0, 2.0, 3.0, (4.0, 5.0))
+    v01, kcl, fc = g.struts()
+    assert v01.tolist() == [[0, 4], [3, 4], [0, 3], [0, 1], [1, 4], [1, 5], [4, 5], [1, 2], [2, 5]]
+    has, kcr = g.torsion()
+    assert has.tolist() == [True, False, False, False, True, True, False, False, False]
+    p, i = g.get_vertices()
+    assert p.tolist() == [[float(x) for x in q] for q in pos] and i.tolist() == idx
+    t = fa.SpringyMesh([[0, 0, 0], [2, 0, 0], [0, 1, 0]], [0, 1, 2], 1.0, 2.0, 3.0, (4.0, 5.0))
+    assert t.get_points()[0, 0] == np.float32(1.0) / np.float32(3.0)
+    v01, kcl, _ = t.struts()
+    assert kcl[0].tolist() == [1.0, 1.5, 2.0]
+
+
+def test_errors_replace_panics():
+    with pytest.raises(ValueError, match="more than two"):  # springy_mesh.rs:339
+        fa.SpringyMesh([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1]], [0, 1, 2, 1, 0, 3, 0, 1, 4],
+                       1.0, 1.0, 1.0)
+    with pytest.raises(IndexError):
+        fa.SpringyMesh([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [0, 1, 5], 1.0, 1.0, 1.0)
+    m = fa.SpringyMesh([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [0, 1, 2], 1.0, 1.0, 1.0)
+    with pytest.raises(IndexError):
+        m.add_strut((0, 7), 1.0, 1.0)
+    with pytest.raises(ValueError):
+        m.set_points(np.zeros((4, 10), np.float32))
+    with pytest.raises((ValueError, IndexError)):
+        fa.CollidableMesh([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [0, 0, 1])  # collidable_mesh.rs:83-86
+    with pytest.raises(ValueError):
+        fa.Cloth(1, 5, 0.1, (0, 0, 0), 1.0, 1, 1, 1, 1, 1, 1, [])
+
+
+def test_obstacle_counts_match_oracle():
+    for p, i in [wl.floor_obstacle(), wl.sphere_obstacle(2.0, (0, 0, 0)), wl.sphere_obstacle(1.0, (1, 2, 3), 3)]:
+        g = fa.CollidableMesh(p, i)
+        o = orc.Obstacle(p, i)
+        assert g.counts() == o.counts()
