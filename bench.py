@@ -35,7 +35,7 @@ def main():
     ap.add_argument("--n", type=int, default=1024, help="cloth side (vertices)")
     ap.add_argument("--math", choices=["exact", "fast"], default="exact")
     ap.add_argument("--kernel", choices=["auto", "grid", "grid_simple", "csr"], default="auto")
-    ap.add_argument("--tile", choices=["default", "16x64", "8x32", "16x16"], default="default")
+    ap.add_argument("--tile", choices=["default", "15x15", "7x31", "16x16"], default="default")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
     args = ap.parse_args()
 
@@ -54,7 +54,7 @@ def main():
             "csr": fa.FP_KERNEL_CSR}[args.kernel]
     sim.set_kernel(kern, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
     if args.tile != "default":
-        sim.set_tile_shape(0, {"16x64": 0, "8x32": 1, "16x16": 2}[args.tile])
+        sim.set_tile_shape(0, {"15x15": 0, "7x31": 1, "16x16": 2}[args.tile])
     N = scene.cloths[0].rows * scene.cloths[0].cols
     dt = scene.config.dt
 

@@ -15,7 +15,7 @@ FP_EULER, FP_RK4 = 0, 1
 FP_PLANE_XY, FP_PLANE_XZ = 0, 1
 FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_KERNEL_GRID_SIMPLE = 0, 1, 2, 3
 FP_MATH_EXACT, FP_MATH_FAST = 0, 1
-FP_TILE_16x64, FP_TILE_8x32, FP_TILE_16x16 = 0, 1, 2
+FP_TILE_15x15, FP_TILE_7x31, FP_TILE_16x16 = 0, 1, 2
 
 # Every symbol declared in include/feriphys_gpu.h, with its ctypes signature.
 _U64 = C.c_uint64

@@ -7,7 +7,7 @@
 //            vertices) and the strut constants (rest length, stiffness, damping) per column pair,
 //            row pair and quad; a Cloth::new grid's rest lengths depend only on those;
 //   phase 2  every vertex gathers its struts (from LDS state and the constant tables) and its faces
-//            (from the phase-1 forces plus its own corner angle) in the reference's order,
+//            (phase-1 force times its own corner's angle / PI) in the reference's order,
 //            integrates, resolves collision and pins, and writes its new state to HBM.
 // The per-vertex operations and their order are those of grid_kernel, so the EXACT form returns
 // the same bits as the reference.  HBM traffic: the state read once (plus halo rows that the L2
@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const Cl
   constexpr int TH = TR + 4, TW = TC + 4;  // staged rows/cols: r0-2 .. r0+TR+1, c0-2 .. c0+TC+1
   constexpr int QR = TR + 1, QC = TC + 1;  // quads (r', c') with r' in [r0-1, r0+TR), c' in [c0-1, c0+TC)
   __shared__ float s_st[6][TH][TW];
-  __shared__ float s_face[2][QR][QC][3];
+  __shared__ float s_face[2][QR][QC][3];  // drag + lift
   __shared__ StrutK s_shear[QR][QC];
   __shared__ StrutK s_h[TC + 1], s_v[TR + 1], s_bh[TC + 2], s_bv[TR + 2];
 
@@ -129,6 +129,7 @@ __global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const Cl
     if (c >= 2 && r <= R - 3) strut(r, c - 2, s_bh[qj]);
     if (r <= R - 3 && c <= C - 3) { strut(r, c + 2, s_bh[qj + 2]); strut(r + 2, c, s_bv[qi + 2]); }
     // faces in face-index order: A(r-1,c) A(r,c-1) A(r,c) B(r-1,c-1) B(r-1,c) B(r,c-1)
+    // this vertex's corner weight (angle / PI) times the face's drag + lift
     auto face = [&](int fs, int i2, int j2, float w) {
       F = add(F, f3{w * s_face[fs][i2][j2][0], w * s_face[fs][i2][j2][1], w * s_face[fs][i2][j2][2]});
     };
@@ -167,8 +168,8 @@ template <class M>
 hipError_t launch_math(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n, int shape,
                        int mode, hipStream_t s) {
   switch (shape) {
-    case TILE_16x64: return launch_shape<M, 16, 64>(a, cloths, tiles, n, mode, s);
-    case TILE_8x32: return launch_shape<M, 8, 32>(a, cloths, tiles, n, mode, s);
+    case TILE_15x15: return launch_shape<M, 15, 15>(a, cloths, tiles, n, mode, s);
+    case TILE_7x31: return launch_shape<M, 7, 31>(a, cloths, tiles, n, mode, s);
     case TILE_16x16: return launch_shape<M, 16, 16>(a, cloths, tiles, n, mode, s);
     default: return hipErrorInvalidValue;
   }

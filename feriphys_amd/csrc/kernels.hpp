@@ -51,10 +51,12 @@ struct TileRange {
   uint32_t mesh, r0, c0, shape;
 };
 
-// Tile shapes compiled for the tiled grid kernel (rows x columns per 256-thread workgroup).
-enum TileShape { TILE_16x64 = 0, TILE_8x32 = 1, TILE_16x16 = 2, N_TILE_SHAPES = 3 };
-constexpr int kTileRows[N_TILE_SHAPES] = {16, 8, 16};
-constexpr int kTileCols[N_TILE_SHAPES] = {64, 32, 16};
+// Tile shapes compiled for the tiled grid kernel (rows x columns per 256-thread workgroup).  A
+// TR x TC tile touches (TR+1) x (TC+1) quads; 15x15 and 7x31 make that exactly 256, one quad (two
+// faces) per thread in the face phase.
+enum TileShape { TILE_15x15 = 0, TILE_7x31 = 1, TILE_16x16 = 2, N_TILE_SHAPES = 3 };
+constexpr int kTileRows[N_TILE_SHAPES] = {15, 7, 16};
+constexpr int kTileCols[N_TILE_SHAPES] = {15, 31, 16};
 
 hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
                       bool exact, hipStream_t s);

@@ -110,15 +110,15 @@ struct MeshSlot {
   fp::StepParams params;
   bool uniform_mass = true;
   int kernel = MK_CSR;
-  int tile_shape = fp::TILE_16x64;
+  int tile_shape = fp::TILE_16x16;
   uint32_t blk_begin = 0, blk_end = 0;
   uint32_t tile_begin = 0, tile_end = 0;  // range in fp_sim::mesh_tiles
 };
 
-// Default tile shape for a cloth of this size (measured on the 1024 x 1024 cloth: 16x16 and 8x32
-// tiles run at 7 workgroups per CU, 16x64 at 2; 8x32 wastes less on cloths under 16 rows).
+// Default tile shape for a cloth of this size: 16x16 (fastest measured on the 1024 x 1024 cloth,
+// one vertex per thread); 7x31 for short wide cloths.
 int default_tile_shape(uint32_t rows, uint32_t cols) {
-  if (rows < 16 && cols >= 32) return fp::TILE_8x32;
+  if (rows < 15 && cols >= 31) return fp::TILE_7x31;
   return fp::TILE_16x16;
 }
 

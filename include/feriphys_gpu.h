@@ -189,10 +189,9 @@ int fp_sim_clear_forces(fp_sim* sim, uint64_t mesh);
 #define FP_MATH_FAST 1
 int fp_sim_set_kernel(fp_sim* sim, int32_t kernel, int32_t math);
 /* Tile shape of the LDS-tiled grid kernel for one cloth mesh (rows x columns per workgroup):
- * FP_TILE_16x64, FP_TILE_8x32 or FP_TILE_16x16.  The default is chosen from the cloth's size;
- * fp_sim_autotune times the shapes and keeps the fastest per mesh-size class. */
-#define FP_TILE_16x64 0
-#define FP_TILE_8x32 1
+ * FP_TILE_15x15, FP_TILE_7x31 or FP_TILE_16x16.  The default is chosen from the cloth's size. */
+#define FP_TILE_15x15 0
+#define FP_TILE_7x31 1
 #define FP_TILE_16x16 2
 int fp_sim_set_tile_shape(fp_sim* sim, uint64_t mesh, int32_t shape);
 int fp_sim_get_tile_shape(const fp_sim* sim, uint64_t mesh, int32_t* shape);
