@@ -68,6 +68,7 @@ SIGNATURES = {
     "fp_sim_set_config": (C.c_int, [_P, C.POINTER(FpConfig)]),
     "fp_sim_get_config": (C.c_int, [_P, C.POINTER(FpConfig)]),
     "fp_sim_set_mesh_config": (C.c_int, [_P, _U64, C.POINTER(FpConfig)]),
+    "fp_sim_set_mesh_configs": (C.c_int, [_P, C.POINTER(FpConfig), _U64]),
     "fp_sim_step": (C.c_int, [_P, _U64, _FP]),
     "fp_sim_step_async": (C.c_int, [_P, _U64]),
     "fp_sim_synchronize": (C.c_int, [_P]),
@@ -83,7 +84,10 @@ SIGNATURES = {
     "fp_sim_set_kernel": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "fp_sim_set_tile_shape": (C.c_int, [_P, _U64, C.c_int32]),
     "fp_sim_get_tile_shape": (C.c_int, [_P, _U64, C.POINTER(C.c_int32)]),
+    "fp_sim_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int32)]),
+    "fp_sim_step_graph": (C.c_int, [_P, _U64, C.c_uint32]),
     "fp_sim_time_steps": (C.c_int, [_P, _U64, _FP]),
+    "fp_sim_time_steps_graph": (C.c_int, [_P, _U64, C.c_uint32, _FP]),
     "fp_sim_free": (None, [_P]),
 }
 

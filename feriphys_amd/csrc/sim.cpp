@@ -70,6 +70,11 @@ template <class T> struct DBuf {
     if (e != hipSuccess || count == 0) return e;
     return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
   }
+  // Like upload, but keeps the allocation (and so the device address) when the size is unchanged.
+  hipError_t assign(const T* h, size_t count) {
+    if (count != n || p == nullptr) return upload(h, count);
+    return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
+  }
 };
 
 // Duration::from_secs_f32(dt).as_secs_f32() (springy_mesh.rs:442; springy/simulation.rs:158).
@@ -153,9 +158,20 @@ struct fp_sim {
   int32_t kernel = FP_KERNEL_AUTO, math = FP_MATH_EXACT;
   DBuf<float> tmp_rec;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // captured step chunks, one per state-buffer parity; valid while `epoch` is unchanged
+  uint64_t epoch = 1;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  uint64_t graph_epoch[2] = {0, 0};
+  uint32_t graph_chunk[2] = {0, 0};
+
+  void drop_graphs() {
+    for (int k = 0; k < 2; ++k)
+      if (graph[k]) { (void)hipGraphExecDestroy(graph[k]); graph[k] = nullptr; }
+  }
 
   ~fp_sim() {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (stream) (void)hipStreamDestroy(stream);
@@ -225,7 +241,8 @@ void refresh_mass(MeshSlot& ms, const float* rec, uint64_t n) {
 int upload_params(fp_sim* s) {
   std::vector<fp::StepParams> p;
   for (auto& m : s->meshes) p.push_back(m.params);
-  HIP_TRY(s->params.upload(p.data(), p.size()));
+  HIP_TRY(s->params.assign(p.data(), p.size()));
+  ++s->epoch;
   return FP_OK;
 }
 
@@ -280,6 +297,7 @@ int build_block_lists(fp_sim* s) {
     HIP_TRY(s->tiles[sh].upload(o.data(), o.size()));
   }
   HIP_TRY(s->mesh_tiles.upload(all_tiles.data(), all_tiles.size()));
+  ++s->epoch;
   return FP_OK;
 }
 
@@ -430,6 +448,43 @@ int run_steps(fp_sim* s, uint64_t n) {
     s->forces_dirty = false;
   }
   return FP_OK;
+}
+
+// Enqueues n steps, replaying captured `chunk`-step graphs where possible.  A step that starts from
+// non-zero force records runs eagerly first (the captured launches assume clear forces).
+int run_steps_graph(fp_sim* s, uint64_t n, uint32_t chunk) {
+  if (chunk == 0) return run_steps(s, n);
+  if (s->forces_dirty && n > 0) {
+    if (int e = run_steps(s, 1)) return e;
+    --n;
+  }
+  while (n >= chunk) {
+    const int par = s->cur;
+    if (!s->graph[par] || s->graph_epoch[par] != s->epoch || s->graph_chunk[par] != chunk) {
+      if (s->graph[par]) { HIP_TRY(hipGraphExecDestroy(s->graph[par])); s->graph[par] = nullptr; }
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      const int e = run_steps(s, chunk);
+      const hipError_t ee = hipStreamEndCapture(s->stream, &g);
+      s->cur = par;  // capture records the launches without running them
+      if (e) return e;
+      if (ee != hipSuccess) return fail(FP_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+      const hipError_t ie = hipGraphInstantiate(&s->graph[par], g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return fail(FP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+      s->graph_epoch[par] = s->epoch;
+      s->graph_chunk[par] = chunk;
+    }
+    HIP_TRY(hipGraphLaunch(s->graph[par], s->stream));
+    s->cur = par ^ (int)(chunk & 1u);
+    n -= chunk;
+  }
+  return run_steps(s, n);
+}
+
+int size_class(const fp::ClothDesc& d) {
+  const uint32_t m = std::max(d.rows, d.cols);
+  return m < 16 ? 0 : m < 32 ? 1 : m < 64 ? 2 : 3;
 }
 
 }  // namespace
@@ -781,6 +836,91 @@ int fp_sim_set_mesh_config(fp_sim* s, uint64_t mesh, const fp_config* c) {
   m.params = make_params(m.cfg, m.params.mass, m.uniform_mass);
   HIP_TRY(hipStreamSynchronize(s->stream));
   return upload_params(s);
+}
+
+int fp_sim_set_mesh_configs(fp_sim* s, const fp_config* c, uint64_t n) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (n != s->meshes.size()) return fail(FP_EINVAL, "one config per mesh expected");
+  for (uint64_t i = 0; i < n; ++i)
+    if (int e = check_config(&c[i])) return e;
+  for (uint64_t i = 0; i < n; ++i) {
+    MeshSlot& m = s->meshes[i];
+    m.cfg = c[i];
+    m.params = make_params(m.cfg, m.params.mass, m.uniform_mass);
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return upload_params(s);
+}
+
+int fp_sim_step_graph(fp_sim* s, uint64_t n, uint32_t chunk) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (int e = run_steps_graph(s, n, chunk)) return e;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_sim_time_steps_graph(fp_sim* s, uint64_t n, uint32_t chunk, float* ms) {
+  if (!s || !ms) return fail(FP_EINVAL, "NULL argument");
+  HIP_TRY(hipEventRecord(s->ev0, s->stream));
+  if (int e = run_steps_graph(s, n, chunk)) return e;
+  HIP_TRY(hipEventRecord(s->ev1, s->stream));
+  HIP_TRY(hipEventSynchronize(s->ev1));
+  HIP_TRY(hipEventElapsedTime(ms, s->ev0, s->ev1));
+  return FP_OK;
+}
+
+int fp_sim_autotune(fp_sim* s, uint32_t steps, int32_t* chosen_out) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (steps == 0) steps = 1;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  // snapshot the state (and force records) so every trial starts from it and it is restored
+  DBuf<float> snap[9];
+  const int c0 = s->cur;
+  for (int k = 0; k < 9; ++k) {
+    const float* src = k < 6 ? s->state[c0][k].p : s->force[k - 6].p;
+    HIP_TRY(snap[k].alloc(s->N));
+    if (s->N) HIP_TRY(hipMemcpy(snap[k].p, src, s->N * sizeof(float), hipMemcpyDeviceToDevice));
+  }
+  const bool dirty0 = s->forces_dirty;
+  auto restore = [&]() -> int {
+    s->cur = c0;
+    for (int k = 0; k < 9; ++k) {
+      float* dst = k < 6 ? s->state[c0][k].p : s->force[k - 6].p;
+      if (s->N) HIP_TRY(hipMemcpy(dst, snap[k].p, s->N * sizeof(float), hipMemcpyDeviceToDevice));
+    }
+    s->forces_dirty = dirty0;
+    return FP_OK;
+  };
+  int32_t chosen[4] = {-1, -1, -1, -1};
+  for (int cls = 0; cls < 4; ++cls) {
+    std::vector<MeshSlot*> members;
+    for (auto& m : s->meshes)
+      if (m.kernel == MK_GRID_TILED && size_class(m.host.cloth) == cls) members.push_back(&m);
+    if (members.empty()) continue;
+    float best_ms = INFINITY;
+    int best = members[0]->tile_shape;
+    for (int shape = 0; shape < fp::N_TILE_SHAPES; ++shape) {
+      for (auto* m : members) m->tile_shape = shape;
+      if (int e = build_block_lists(s)) return e;
+      if (int e = restore()) return e;
+      if (int e = run_steps(s, 2)) return e;  // warm-up
+      float ms = 0.0f;
+      HIP_TRY(hipEventRecord(s->ev0, s->stream));
+      if (int e = run_steps(s, steps)) return e;
+      HIP_TRY(hipEventRecord(s->ev1, s->stream));
+      HIP_TRY(hipEventSynchronize(s->ev1));
+      HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+      if (ms < best_ms) { best_ms = ms; best = shape; }
+    }
+    for (auto* m : members) m->tile_shape = best;
+    chosen[cls] = best;
+  }
+  if (int e = build_block_lists(s)) return e;
+  if (int e = restore()) return e;
+  HIP_TRY(hipDeviceSynchronize());
+  if (chosen_out)
+    for (int k = 0; k < 4; ++k) chosen_out[k] = chosen[k];
+  return FP_OK;
 }
 
 int fp_sim_step_async(fp_sim* s, uint64_t n) {

@@ -246,6 +246,23 @@ class Simulation(_Handle):
         c = cfg.to_c()
         check(lib().fp_sim_set_mesh_config(self.handle, mesh, C.byref(c)))
 
+    def set_mesh_configs(self, cfgs):
+        arr = (_lib.FpConfig * len(cfgs))(*[c.to_c() for c in cfgs])
+        check(lib().fp_sim_set_mesh_configs(self.handle, arr, len(cfgs)))
+
+    def autotune(self, steps_per_trial: int = 20):
+        out = (C.c_int32 * 4)()
+        check(lib().fp_sim_autotune(self.handle, steps_per_trial, out))
+        return list(out)
+
+    def step_graph(self, n_steps: int, chunk: int = 100):
+        check(lib().fp_sim_step_graph(self.handle, n_steps, chunk))
+
+    def time_steps_graph(self, n_steps: int, chunk: int = 100) -> float:
+        ms = C.c_float()
+        check(lib().fp_sim_time_steps_graph(self.handle, n_steps, chunk, C.byref(ms)))
+        return ms.value
+
     def step(self, n_steps: int = 1) -> float:
         """Simulation::step (springy/simulation.rs:27-45); returns Duration(dt) in seconds."""
         ts = C.c_float()

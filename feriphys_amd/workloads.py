@@ -191,21 +191,79 @@ def config1(n: int = 1024, steps: int = 600, seed: int = 0, subdivisions: int = 
     return Scene([cl], cfg, [sphere], 1e-3, seed, steps, f"cloth{n}x{n}+sphere")
 
 
+def config2(n_cloths: int = 4096, seed: int = 0, smin: int = 8, smax: int = 256, steps: int = 100) -> Scene:
+    """n_cloths independent grid cloths: rows and cols each drawn log-uniform in [smin, smax]
+    (about 21M vertices for 4096); per-cloth stiffness scale uniform in [0.5, 2] on the config-0
+    tensile/shear/bend constants (damping unchanged); pins: 1-4 random corners or one full edge;
+    per-cloth wind with direction uniform on the upper hemisphere and magnitude uniform in [0, 3]
+    m/s; config-0 mass, spacing, gravity, drag and lift; RK4 at dt = 1/600 s.  Each cloth is a
+    mesh with its own Config (one reference Simulation per cloth)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(smin), math.log(smax + 1)
+    cloths, winds = [], []
+    for k in range(n_cloths):
+        R = int(min(smax, max(smin, math.floor(math.exp(rng.uniform(lo, hi))))))
+        C = int(min(smax, max(smin, math.floor(math.exp(rng.uniform(lo, hi))))))
+        sk = float(rng.uniform(0.5, 2.0))
+        corners = [0, C - 1, (R - 1) * C, R * C - 1]
+        if rng.uniform() < 0.5:
+            npin = int(rng.integers(1, 5))
+            pins = sorted(int(x) for x in rng.choice(corners, npin, replace=False))
+        else:
+            edge = int(rng.integers(0, 4))
+            pins = ([c for c in range(C)] if edge == 0 else [(R - 1) * C + c for c in range(C)] if edge == 1
+                    else [r * C for r in range(R)] if edge == 2 else [r * C + C - 1 for r in range(R)])
+        cloths.append(cloth_spec(R, C, tensile=(500.0 * sk, 2.0), shear=(300.0 * sk, 1.0), bend=(50.0 * sk, 0.5),
+                                 pins=pins))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        d[1] = abs(d[1])
+        winds.append(tuple(float(F32(x)) for x in d * rng.uniform(0.0, 3.0)))
+    base = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.01), (0.0, 0.0, 0.0), 0.2, 0.5, 0.95, 0.3).f32()
+    sc = Scene(cloths, base, [], 1e-3, seed, steps, f"{n_cloths} cloths")
+    sc.mesh_winds = winds
+    return sc
+
+
+def mesh_configs(scene: Scene):
+    """The Config of every mesh: the scene's, with the per-cloth wind where the scene has one."""
+    winds = getattr(scene, "mesh_winds", None)
+    out = []
+    for k in range(len(scene.cloths)):
+        c = scene.config
+        w = winds[k] if winds is not None else c.wind
+        out.append(ConfigSpec(c.integration, c.dt, c.gravity, w, c.lift_coefficient, c.drag_coefficient,
+                              c.coefficient_of_restitution, c.coefficient_of_friction))
+    return out
+
+
 # ---- building a scene on either side ----------------------------------------------------------
 
-def build_gpu(scene: Scene):
-    """The scene as a feriphys_amd Simulation (records jittered, config set)."""
+def build_gpu(scene: Scene, threads: int = 8):
+    """The scene as a feriphys_amd Simulation (records jittered, configs set).  Cloths are built on
+    a thread pool (the host builder runs outside the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+
     from . import springy as sp
 
-    meshes = []
-    for k, c in enumerate(scene.cloths):
+    def make(k):
+        c = scene.cloths[k]
         cl = sp.Cloth(*c.args(), plane=c.plane)
         if scene.jitter:
             cl.mesh.set_points(jitter_records(cl.mesh.get_points(), c.pins, scene.jitter, scene.seed + k))
-        meshes.append(cl.mesh)
+        return cl.mesh
+
+    if len(scene.cloths) > 1 and threads > 1:
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            meshes = list(ex.map(make, range(len(scene.cloths))))
+    else:
+        meshes = [make(k) for k in range(len(scene.cloths))]
     obs = [sp.CollidableMesh(p, i) for p, i in scene.obstacles]
     sim = sp.Simulation(meshes, obs)
-    c = scene.config
-    sim.config = sp.Config(sp.Integration(c.integration), c.dt, c.gravity, c.wind, c.lift_coefficient,
-                           c.drag_coefficient, c.coefficient_of_restitution, c.coefficient_of_friction)
+    to_cfg = lambda c: sp.Config(sp.Integration(c.integration), c.dt, c.gravity, c.wind, c.lift_coefficient,
+                                 c.drag_coefficient, c.coefficient_of_restitution, c.coefficient_of_friction)
+    if getattr(scene, "mesh_winds", None) is not None:
+        sim.set_mesh_configs([to_cfg(c) for c in mesh_configs(scene)])
+    else:
+        sim.config = to_cfg(scene.config)
     return sim

@@ -148,6 +148,8 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
 int fp_sim_set_config(fp_sim* sim, const fp_config* config);
 int fp_sim_get_config(const fp_sim* sim, fp_config* config);
 int fp_sim_set_mesh_config(fp_sim* sim, uint64_t mesh, const fp_config* config);
+/* One Config per mesh for all meshes at once (configs[n_meshes]); one upload. */
+int fp_sim_set_mesh_configs(fp_sim* sim, const fp_config* configs, uint64_t n_meshes);
 /* Simulation::step (springy/simulation.rs:27-45), n_steps times; blocks until done.
  * timestep_out (may be NULL) receives Duration::from_secs_f32(dt).as_secs_f32(). */
 int fp_sim_step(fp_sim* sim, uint64_t n_steps, float* timestep_out);
@@ -195,9 +197,20 @@ int fp_sim_set_kernel(fp_sim* sim, int32_t kernel, int32_t math);
 #define FP_TILE_16x16 2
 int fp_sim_set_tile_shape(fp_sim* sim, uint64_t mesh, int32_t shape);
 int fp_sim_get_tile_shape(const fp_sim* sim, uint64_t mesh, int32_t* shape);
+/* Autotuner: meshes on the tiled grid kernel are grouped by size class (longer side < 16, < 32,
+ * < 64, >= 64 vertices); for each class every tile shape is timed over `steps_per_trial` steps
+ * (HIP events on the simulation's stream, state restored afterwards) and the fastest is kept.
+ * chosen_out[4] (may be NULL) receives the shape per class (-1: no mesh in the class). */
+int fp_sim_autotune(fp_sim* sim, uint32_t steps_per_trial, int32_t* chosen_out);
+/* n_steps of Simulation::step replayed from HIP graphs that capture `chunk` steps each (the
+ * launch sequence of one chunk, captured once per state-buffer parity and reused until the
+ * simulation's kernels, configs or tiles change); any remainder runs eagerly.  Blocks. */
+int fp_sim_step_graph(fp_sim* sim, uint64_t n_steps, uint32_t chunk);
 /* Runs n_steps on the simulation's stream bracketed by HIP events recorded on that stream;
  * kernel_ms receives the event-measured time of the whole run (device time, inputs resident). */
 int fp_sim_time_steps(fp_sim* sim, uint64_t n_steps, float* kernel_ms);
+/* The same through fp_sim_step_graph's captured chunks. */
+int fp_sim_time_steps_graph(fp_sim* sim, uint64_t n_steps, uint32_t chunk, float* kernel_ms);
 void fp_sim_free(fp_sim* sim);
 
 #ifdef __cplusplus

@@ -376,6 +376,29 @@ class Simulation:
         return Mesh(_fn("orc_sim_mesh", self.dtype)(self.h, i), self.dtype, owned=False)
 
 
+def build_scene_per_mesh(scene, dtype=np.float32):
+    """One oracle Simulation per cloth (a reference Simulation holds one Config; per-cloth winds
+    need one each), with the scene's jitter, obstacles and per-mesh configs."""
+    from feriphys_amd.workloads import jitter_records, mesh_configs
+
+    dt = np.dtype(dtype).type
+    sims = []
+    for k, (c, cfg) in enumerate(zip(scene.cloths, mesh_configs(scene))):
+        m = Mesh.cloth(*c.args(), plane=c.plane, dtype=np.float32)
+        rec = m.points()
+        if scene.jitter:
+            rec = jitter_records(rec, c.pins, scene.jitter, scene.seed + k)
+        if dt is not np.float32:
+            m = Mesh.cloth(*c.args(), plane=c.plane, dtype=dt)
+        m.set_points(rec.astype(dt))
+        obs = [Obstacle(np.asarray(p, np.float32).astype(dt), i, dtype=dt) for p, i in scene.obstacles]
+        sim = Simulation([m], obs, dtype=dt)
+        sim.set_config(Config(cfg.integration, cfg.dt, cfg.gravity, cfg.wind, cfg.lift_coefficient,
+                              cfg.drag_coefficient, cfg.coefficient_of_restitution, cfg.coefficient_of_friction))
+        sims.append(sim)
+    return sims
+
+
 def build_scene(scene, dtype=np.float32) -> Simulation:
     """The oracle Simulation of a feriphys_amd.workloads.Scene: the same Cloth::new arguments, the
     same seeded jitter (computed on the f32 records; the f64 run starts from them widened), the
