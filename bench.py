@@ -37,8 +37,9 @@ def main():
     ap.add_argument("--kernel", choices=["auto", "grid", "grid_simple", "csr"], default="auto")
     ap.add_argument("--tile", choices=["default", "15x15", "7x31", "16x16"], default="default")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
-    ap.add_argument("--workload", choices=["cloth", "batched"], default="cloth",
-                    help="cloth: one 1024x1024 cloth + sphere (default); batched: 4096 cloths of uneven sizes")
+    ap.add_argument("--workload", choices=["cloth", "batched", "surfaces"], default="cloth",
+                    help="cloth: one 1024x1024 cloth + sphere (default); batched: 4096 cloths of uneven sizes; "
+                         "surfaces: 64 perturbed icospheres (2.1M vertices) on a ground plane")
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph (batched)")
     args = ap.parse_args()
 
@@ -53,6 +54,8 @@ def main():
 
     if args.workload == "batched":
         return bench_batched(args, np, fa, wl)
+    if args.workload == "surfaces":
+        return bench_surfaces(args, np, fa, wl)
     scene = wl.config1(n=args.n, steps=args.steps)
     sim = wl.build_gpu(scene)
     kern = {"auto": fa.FP_KERNEL_AUTO, "grid": fa.FP_KERNEL_GRID, "grid_simple": fa.FP_KERNEL_GRID_SIMPLE,
@@ -186,6 +189,62 @@ def bench_batched(args, np, fa, wl):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_avg_us": kernel_avg_s * 1e6,
                      "bytes_per_launch": BYTES_PER_VERTEX_STEP * N},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_surfaces(args, np, fa, wl):
+    """64 icospheres of uneven size (levels 4-7, 2,145,408 vertices) on the CSR kernel."""
+    scene = wl.config4()
+    sim = wl.build_surface_gpu(scene)
+    sim.set_kernel(fa.FP_KERNEL_AUTO, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
+    N = sum(m.positions.shape[0] for m in scene.meshes)
+    S = sum(sim.mesh_counts(k)[1] for k in range(len(scene.meshes)))
+    F = sum(sim.mesh_counts(k)[2] for k in range(len(scene.meshes)))
+    dt = scene.config.dt
+    initial = [sim.get_points(k) for k in range(len(scene.meshes))]
+    if args.warmup:
+        sim.step_graph(args.warmup, args.chunk)
+        for k, rec in enumerate(initial):
+            sim.set_points(k, rec)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms = sim.time_steps_graph(args.steps, args.chunk)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    # CSR algorithmic bytes per step: state 48 B/vertex, row pointers 2 x 4 B/vertex, 16 B per
+    # directed strut entry, 4 B per face corner entry, face vertex ids 12 B/face, and the face
+    # kernel's per-face results written once and read per corner (16 + 12 B written, 3 x (16 + 4) read)
+    bytes_step = 48 * N + 8 * N + 16 * 2 * S + 4 * 3 * F + 12 * F + 28 * F + 60 * F
+    kernel_avg_s = kernel_ms * 1e-3 / args.steps
+    achieved = bytes_step / kernel_avg_s / 1e9
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        sub = wl.config4(n_meshes=8, levels=[5, 5, 4, 4, 4, 4, 4, 4])
+        o = orc.build_surface_scene(sub)
+        n_sub = sum(m.positions.shape[0] for m in sub.meshes)
+        t = time.perf_counter()
+        o.step(args.cpu_steps)
+        cpu_wall = time.perf_counter() - t
+        vsps = args.cpu_steps * n_sub / cpu_wall
+        cpu = {"value": vsps / N * dt, "unit": "sim-s/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} steps of 8 icospheres ({n_sub} vertices), scaled by vertex count to "
+                         f"{N} vertices", "vertex_steps_per_s": vsps}
+    line = {
+        "metric": "sim-s/s, irregular meshes (64 icospheres, CSR); %HBM peak",
+        "value": args.steps * dt / wall, "unit": "sim-s/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"plain springy step, 64 radially perturbed icospheres (levels 4-7), {N} vertices, "
+                               f"{S} struts, {F} faces, ground plane, RK4 dt=1/600 s", "math": args.math},
+        "vertex_steps_per_s": args.steps * N / wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_avg_us": kernel_avg_s * 1e6,
+                     "bytes_per_launch": bytes_step},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -45,18 +45,15 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
   f3 F = A.force_init ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
   F = add(F, P.gravity);  // apply_external_point_forces (:527-531)
   for (uint32_t e = T.s_row[i], ee = T.s_row[i + 1]; e < ee; ++e) {  // apply_strut_forces (:533-551)
-    const uint32_t j = T.s_nbr[e];
-    strut_accumulate<M>(F, p, v, ld3(S.x, S.y, S.z, j), ld3(S.vx, S.vy, S.vz, j), T.s_k[e], T.s_c[e], T.s_len[e]);
+    const StrutEnt se = T.s_ent[e];
+    strut_accumulate<M>(F, p, v, ld3(S.x, S.y, S.z, se.nbr), ld3(S.vx, S.vy, S.vz, se.nbr), se.k, se.c, se.len);
   }
-  for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {  // apply_face_forces (:629-654)
+  // apply_face_forces (:629-654): the face kernel has computed each face's drag + lift and corner
+  // weights; this vertex adds weight * force for its faces in face order
+  for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {
     const uint32_t ent = T.f_ent[e];
-    const uint32_t f = ent >> 2, corner = ent & 3u;
-    const uint32_t a = T.f_v[3 * f], b = T.f_v[3 * f + 1], c = T.f_v[3 * f + 2];
-    const f3 p0 = ld3(S.x, S.y, S.z, a), p1 = ld3(S.x, S.y, S.z, b), p2 = ld3(S.x, S.y, S.z, c);
-    const f3 sum = face_force<M>(p0, p1, p2, ld3(S.vx, S.vy, S.vz, a), ld3(S.vx, S.vy, S.vz, b),
-                                 ld3(S.vx, S.vy, S.vz, c), P);
-    const float w = corner == 0 ? corner_weight<M>(p0, p1, p2)
-                                : (corner == 1 ? corner_weight<M>(p1, p2, p0) : corner_weight<M>(p2, p0, p1));
+    const float4 sum = T.face_sum[ent >> 2];
+    const float w = T.face_w[3 * (ent >> 2) + (ent & 3u)];
     F = add(F, f3{w * sum.x, w * sum.y, w * sum.z});
   }
   const bool pin = pinned(A.pins, i);
@@ -66,6 +63,25 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
     return;
   }
   finish_step<M>(A, P, i, m, p, v, F, pin);
+}
+
+// Drag + lift (springy_mesh.rs:631-646) and the three corner weights angle / PI (:647-649) of every
+// face of the CSR meshes, once per face.
+template <class M>
+__global__ void __launch_bounds__(kBlock) face_kernel(StepArgs A, CsrView T, const BlockRange* __restrict__ blocks) {
+  const BlockRange br = blocks[blockIdx.x];
+  const uint32_t f = br.begin + threadIdx.x;
+  if (f >= br.end) return;
+  const StepParams P = A.params[br.mesh];
+  const SoA& S = A.in;
+  const uint32_t a = T.f_v[3 * f], b = T.f_v[3 * f + 1], c = T.f_v[3 * f + 2];
+  const f3 p0 = ld3(S.x, S.y, S.z, a), p1 = ld3(S.x, S.y, S.z, b), p2 = ld3(S.x, S.y, S.z, c);
+  const f3 sum = face_force<M>(p0, p1, p2, ld3(S.vx, S.vy, S.vz, a), ld3(S.vx, S.vy, S.vz, b),
+                               ld3(S.vx, S.vy, S.vz, c), P);
+  T.face_sum[f] = make_float4(sum.x, sum.y, sum.z, 0.0f);
+  T.face_w[3 * f] = corner_weight<M>(p0, p1, p2);
+  T.face_w[3 * f + 1] = corner_weight<M>(p1, p2, p0);
+  T.face_w[3 * f + 2] = corner_weight<M>(p2, p0, p1);
 }
 
 // Grid cloth built by Cloth::new (springy/cloth.rs:20-113).  Vertex (r, c) of an R x C grid.
@@ -188,6 +204,14 @@ hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blo
     if (mode == MODE_STEP) csr_kernel<FastMath, MODE_STEP><<<g, b, 0, s>>>(a, t, blocks);
     else csr_kernel<FastMath, MODE_FORCES><<<g, b, 0, s>>>(a, t, blocks);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_faces(const StepArgs& a, const CsrView& t, const BlockRange* face_blocks, uint32_t n_blocks,
+                        bool exact, hipStream_t s) {
+  if (n_blocks == 0) return hipSuccess;
+  if (exact) face_kernel<ExactMath><<<dim3(n_blocks), dim3(kBlock), 0, s>>>(a, t, face_blocks);
+  else face_kernel<FastMath><<<dim3(n_blocks), dim3(kBlock), 0, s>>>(a, t, face_blocks);
   return hipGetLastError();
 }
 

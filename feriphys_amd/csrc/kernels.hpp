@@ -14,16 +14,21 @@ struct BlockRange {
   uint32_t mesh, begin, end, pad;
 };
 
+// One incident strut of a vertex: the other end and the strut's constants (16 bytes, one load).
+struct StrutEnt {
+  uint32_t nbr;
+  float len, k, c;
+};
+
 // Vertex-centric CSR of a packed set of meshes (global vertex and face ids).
 struct CsrView {
   const uint32_t* s_row;  // N+1: incident struts of each vertex, in strut-index order
-  const uint32_t* s_nbr;
-  const float* s_len;
-  const float* s_k;
-  const float* s_c;
+  const StrutEnt* s_ent;
   const uint32_t* f_row;  // N+1: incident faces of each vertex, in face-index order
   const uint32_t* f_ent;  // (face << 2) | corner
   const uint32_t* f_v;    // 3 global vertex ids per face
+  float4* face_sum;       // per face: drag + lift (x, y, z), written by the face kernel
+  float* face_w;          // per face corner: angle / PI, written by the face kernel
 };
 
 // A Cloth::new grid: enough to rebuild its struts, rest lengths and faces on chip.
@@ -60,6 +65,10 @@ constexpr int kTileCols[N_TILE_SHAPES] = {15, 31, 16};
 
 hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
                       bool exact, hipStream_t s);
+// Drag + lift and corner weights of the faces in `face_blocks` (BlockRange over global face ids),
+// for the CSR kernel that runs after it in the same stream.
+hipError_t launch_faces(const StepArgs& a, const CsrView& t, const BlockRange* face_blocks, uint32_t n_blocks,
+                        bool exact, hipStream_t s);
 hipError_t launch_grid(const StepArgs& a, const ClothView* cloths, const BlockRange* blocks, uint32_t n_blocks,
                        int mode, bool exact, hipStream_t s);
 // LDS-tiled grid kernel over `n_tiles` tiles that all have tile shape `shape`.

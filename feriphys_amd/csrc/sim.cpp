@@ -117,6 +117,7 @@ struct MeshSlot {
   int kernel = MK_CSR;
   int tile_shape = fp::TILE_16x16;
   uint32_t blk_begin = 0, blk_end = 0;
+  uint32_t fblk_begin = 0, fblk_end = 0;  // face blocks (range in fp_sim::h_face_blocks)
   uint32_t tile_begin = 0, tile_end = 0;  // range in fp_sim::mesh_tiles
 };
 
@@ -139,8 +140,15 @@ struct fp_sim {
   bool forces_dirty = false;
   DBuf<float> mass;
   DBuf<uint32_t> pins;
-  DBuf<uint32_t> s_row, s_nbr, f_row, f_ent, f_v;
-  DBuf<float> s_len, s_k, s_c;
+  DBuf<uint32_t> s_row, f_row, f_ent, f_v;
+  DBuf<fp::StrutEnt> s_ent;
+  DBuf<float4> face_sum;
+  DBuf<float> face_w;
+  // face blocks of the meshes on the CSR kernel (BlockRange over global face ids), all meshes'
+  // face blocks (for the split API, indexed per mesh by fblk_begin/fblk_end)
+  DBuf<fp::BlockRange> csr_face_blocks, all_face_blocks;
+  std::vector<fp::BlockRange> h_face_blocks;
+  uint32_t n_csr_face_blocks = 0;
   DBuf<fp::ObsFace> obs_faces;
   DBuf<uint32_t> cell_start[4], cell_faces[4], always;
   uint64_t n_always = 0, n_binned[4] = {0, 0, 0, 0};
@@ -193,7 +201,7 @@ struct fp_sim {
     return a;
   }
   fp::CsrView csr() const {
-    return fp::CsrView{s_row.p, s_nbr.p, s_len.p, s_k.p, s_c.p, f_row.p, f_ent.p, f_v.p};
+    return fp::CsrView{s_row.p, s_ent.p, f_row.p, f_ent.p, f_v.p, face_sum.p, face_w.p};
   }
 };
 
@@ -268,7 +276,7 @@ std::vector<fp::TileRange> xcd_order(const std::vector<fp::TileRange>& t) {
 }
 
 int build_block_lists(fp_sim* s) {
-  std::vector<fp::BlockRange> g, c;
+  std::vector<fp::BlockRange> g, c, cf;
   std::vector<fp::TileRange> by_shape[fp::N_TILE_SHAPES], all_tiles;
   for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
     MeshSlot& ms = s->meshes[mi];
@@ -286,7 +294,11 @@ int build_block_lists(fp_sim* s) {
       continue;
     }
     for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
+    if (ms.kernel == MK_CSR)
+      for (uint32_t b = ms.fblk_begin; b < ms.fblk_end; ++b) cf.push_back(s->h_face_blocks[b]);
   }
+  s->n_csr_face_blocks = (uint32_t)cf.size();
+  HIP_TRY(s->csr_face_blocks.upload(cf.data(), cf.size()));
   s->n_grid_blocks = (uint32_t)g.size();
   s->n_csr_blocks = (uint32_t)c.size();
   HIP_TRY(s->grid_blocks.upload(g.data(), g.size()));
@@ -443,6 +455,7 @@ int run_steps(fp_sim* s, uint64_t n) {
       HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->tiles[sh].p, s->n_tiles[sh], sh, fp::MODE_STEP, exact,
                                     s->stream));
     HIP_TRY(fp::launch_grid(a, s->cloths.p, s->grid_blocks.p, s->n_grid_blocks, fp::MODE_STEP, exact, s->stream));
+    HIP_TRY(fp::launch_faces(a, s->csr(), s->csr_face_blocks.p, s->n_csr_face_blocks, exact, s->stream));
     HIP_TRY(fp::launch_csr(a, s->csr(), s->csr_blocks.p, s->n_csr_blocks, fp::MODE_STEP, exact, s->stream));
     s->cur ^= 1;
     s->forces_dirty = false;
@@ -747,17 +760,15 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
       }
     }
     for (uint64_t i = 0; i < N; ++i) { srow[i + 1] += srow[i]; frow[i + 1] += frow[i]; }
-    std::vector<uint32_t> snbr(srow[N]), fent(frow[N]);
-    std::vector<float> slen(srow[N]), sk(srow[N]), sc(srow[N]);
+    std::vector<fp::StrutEnt> sent(srow[N]);
+    std::vector<uint32_t> fent(frow[N]);
     std::vector<uint32_t> sfill(srow.begin(), srow.end() - 1), ffill(frow.begin(), frow.end() - 1);
     for (auto& ms_ : s->meshes) {
       const auto& h = ms_.host;
       for (uint64_t k = 0; k < h.n_struts(); ++k) {
         uint32_t a = (uint32_t)(ms_.vbase + h.s_v0[k]), b = (uint32_t)(ms_.vbase + h.s_v1[k]);
-        uint32_t ea = sfill[a]++;
-        snbr[ea] = b; slen[ea] = h.s_len[k]; sk[ea] = h.s_k[k]; sc[ea] = h.s_c[k];
-        uint32_t eb = sfill[b]++;
-        snbr[eb] = a; slen[eb] = h.s_len[k]; sk[eb] = h.s_k[k]; sc[eb] = h.s_c[k];
+        sent[sfill[a]++] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
+        sent[sfill[b]++] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
       }
       for (uint64_t t = 0; t < h.n_faces(); ++t)
         for (uint32_t j = 0; j < 3; ++j) {
@@ -766,10 +777,9 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
         }
     }
     HIP_TRY(s->s_row.upload(srow.data(), srow.size()));
-    HIP_TRY(s->s_nbr.upload(snbr.data(), snbr.size()));
-    HIP_TRY(s->s_len.upload(slen.data(), slen.size()));
-    HIP_TRY(s->s_k.upload(sk.data(), sk.size()));
-    HIP_TRY(s->s_c.upload(sc.data(), sc.size()));
+    HIP_TRY(s->s_ent.upload(sent.data(), sent.size()));
+    HIP_TRY(s->face_sum.alloc(F));
+    HIP_TRY(s->face_w.alloc(3 * F));
     HIP_TRY(s->f_row.upload(frow.data(), frow.size()));
     HIP_TRY(s->f_ent.upload(fent.data(), fent.size()));
     HIP_TRY(s->f_v.upload(fv.data(), fv.size()));
@@ -797,9 +807,16 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
         s->h_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.vbase + b),
                                              (uint32_t)(ms_.vbase + std::min<uint64_t>(ms_.host.n, b + fp::kBlock)), 0u});
       ms_.blk_end = (uint32_t)s->h_blocks.size();
+      ms_.fblk_begin = (uint32_t)s->h_face_blocks.size();
+      const uint64_t nf = ms_.host.n_faces();
+      for (uint64_t b = 0; b < nf; b += fp::kBlock)
+        s->h_face_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.fbase + b),
+                                                  (uint32_t)(ms_.fbase + std::min<uint64_t>(nf, b + fp::kBlock)), 0u});
+      ms_.fblk_end = (uint32_t)s->h_face_blocks.size();
     }
     HIP_TRY(s->cloths.upload(cv.data(), cv.size()));
     HIP_TRY(s->all_blocks.upload(s->h_blocks.data(), s->h_blocks.size()));
+    HIP_TRY(s->all_face_blocks.upload(s->h_face_blocks.data(), s->h_face_blocks.size()));
   }
   std::vector<fp::HostObstacle> obs;
   for (auto& o : os) obs.push_back(std::move(o->o));
@@ -1032,7 +1049,11 @@ int fp_sim_accumulate_forces(fp_sim* s, uint64_t mesh) {
     HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->mesh_tiles.p + m.tile_begin, m.tile_end - m.tile_begin,
                                   m.tile_shape, fp::MODE_FORCES, exact, s->stream));
   else if (m.kernel == MK_GRID_SIMPLE) HIP_TRY(fp::launch_grid(a, s->cloths.p, blk, nb, fp::MODE_FORCES, exact, s->stream));
-  else HIP_TRY(fp::launch_csr(a, s->csr(), blk, nb, fp::MODE_FORCES, exact, s->stream));
+  else {
+    HIP_TRY(fp::launch_faces(a, s->csr(), s->all_face_blocks.p + m.fblk_begin, m.fblk_end - m.fblk_begin, exact,
+                             s->stream));
+    HIP_TRY(fp::launch_csr(a, s->csr(), blk, nb, fp::MODE_FORCES, exact, s->stream));
+  }
   s->forces_dirty = true;
   HIP_TRY(hipStreamSynchronize(s->stream));
   return FP_OK;

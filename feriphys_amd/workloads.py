@@ -237,6 +237,116 @@ def mesh_configs(scene: Scene):
     return out
 
 
+def icosphere_fast(subdivisions: int):
+    """icosphere() with the subdivision vectorised (edge midpoints by numpy unique): vertices
+    (float64, unit radius) and outward counter-clockwise triangles (int64)."""
+    v, f = icosphere(0)
+    for _ in range(subdivisions):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.sort(e, axis=1)
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid])
+        m = inv.reshape(3, -1).T + base  # midpoints of (v0v1, v1v2, v2v0) per face
+        a, b, c = f[:, 0], f[:, 1], f[:, 2]
+        ab, bc, ca = m[:, 0], m[:, 1], m[:, 2]
+        f = np.concatenate([np.stack([a, ab, ca], 1), np.stack([b, bc, ab], 1), np.stack([c, ca, bc], 1),
+                            np.stack([ab, bc, ca], 1)])
+    return v, f
+
+
+@dataclass
+class SurfaceSpec:
+    """A SpringyMesh from a triangle list (SpringyMesh::new) plus bending struts across every
+    shared edge (add_strut between the two opposite vertices, in strut order)."""
+    positions: np.ndarray  # (n, 3) f32
+    indices: np.ndarray    # (3F,) uint64
+    mass: float            # total
+    k: float
+    c: float
+    bk: float
+    bc: float
+
+
+def bend_pairs(v01: np.ndarray, faces: np.ndarray, face_verts: np.ndarray) -> np.ndarray:
+    """For every strut with two adjacent faces (strut order), the two vertices opposite it."""
+    both = (faces[:, 0] >= 0) & (faces[:, 1] >= 0)
+    s = v01[both].astype(np.int64)
+    out = []
+    for side in (0, 1):
+        fv = face_verts[faces[both, side]].astype(np.int64)
+        is_end = (fv == s[:, :1]) | (fv == s[:, 1:2])
+        out.append(fv[~is_end].reshape(-1))
+    return np.stack(out, 1)
+
+
+def config4(n_meshes: int = 64, seed: int = 0, levels=None) -> "SurfaceScene":
+    """Seeded icospheres, subdivided to ~2M vertices over 64 meshes of uneven size (levels 4-7),
+    vertices perturbed radially by +-2%; tensile struts on triangle edges and bending struts across
+    shared edges; 0.005 kg per vertex; per-triangle wind/drag/lift; one ground-plane collider at
+    y = 0.  Spring constants per unit strain at the mesh's mean edge length (tensile 100 N/m,
+    0.3 N s/m; bend 10 N/m, 0.05 N s/m), chosen so that the explicit step is stable at dt = 1/600 s
+    for this mass and a vertex degree of 5-7."""
+    rng = np.random.default_rng(seed)
+    if levels is None:
+        # 8 x 163,842 + 14 x 40,962 + 20 x 10,242 + 22 x 2,562 = 2,145,408 vertices (levels 7, 6, 5, 4)
+        levels = [7] * 8 + [6] * 14 + [5] * 20 + [4] * 22
+        levels = levels[:n_meshes]
+    levels = list(levels)
+    rng.shuffle(levels)
+    meshes = []
+    cache = {}
+    for k, lv in enumerate(levels):
+        if lv not in cache:
+            cache[lv] = icosphere_fast(lv)
+        v, f = cache[lv]
+        r = 0.004 * 2 ** lv * rng.uniform(0.8, 1.25)  # edge length ~5 mm x 0.8..1.25
+        noise = rng.uniform(-0.02, 0.02, size=(v.shape[0], 1))
+        centre = np.array([(k % 8) * 3.0, r + rng.uniform(0.05, 0.2), (k // 8) * 3.0])
+        pos = (v * r * (1.0 + noise) + centre).astype(F32)
+        e = np.linalg.norm(pos[f[:, 0]].astype(np.float64) - pos[f[:, 1]], axis=1).mean()
+        meshes.append(SurfaceSpec(pos, f.reshape(-1).astype(np.uint64), float(F32(0.005 * v.shape[0])),
+                                  per_strain(100.0, e), per_strain(0.3, e), per_strain(10.0, 2 * e),
+                                  per_strain(0.05, 2 * e)))
+    ground = (np.array([[-1e3, 0.0, 1e3], [1e3, 0.0, 1e3], [1e3, 0.0, -1e3], [-1e3, 0.0, -1e3]], F32),
+              np.array([0, 1, 2, 0, 2, 3], np.uint64))
+    cfg = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.005), (1.0, 0.0, 0.5), 0.2, 0.5, 0.3, 0.2).f32()
+    return SurfaceScene(meshes, cfg, [ground], f"{len(meshes)} icospheres")
+
+
+@dataclass
+class SurfaceScene:
+    meshes: List[SurfaceSpec]
+    config: ConfigSpec
+    obstacles: list
+    name: str = ""
+
+
+def build_surface_gpu(scene: "SurfaceScene", threads: int = 8):
+    """The SurfaceScene as a feriphys_amd Simulation."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from . import springy as sp
+
+    def make(spec):
+        m = sp.SpringyMesh(spec.positions, spec.indices, spec.mass, spec.k, spec.c)
+        v01, _, faces = m.struts()
+        _, fv = m.faces()
+        for a, b in bend_pairs(v01, faces, fv):
+            m.add_strut((int(a), int(b)), spec.bk, spec.bc)
+        return m
+
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        meshes = list(ex.map(make, scene.meshes))
+    sim = sp.Simulation(meshes, [sp.CollidableMesh(p, i) for p, i in scene.obstacles])
+    c = scene.config
+    sim.config = sp.Config(sp.Integration(c.integration), c.dt, c.gravity, c.wind, c.lift_coefficient,
+                           c.drag_coefficient, c.coefficient_of_restitution, c.coefficient_of_friction)
+    return sim
+
+
 # ---- building a scene on either side ----------------------------------------------------------
 
 def build_gpu(scene: Scene, threads: int = 8):

@@ -399,6 +399,28 @@ def build_scene_per_mesh(scene, dtype=np.float32):
     return sims
 
 
+def build_surface_scene(scene, dtype=np.float32) -> Simulation:
+    """The oracle Simulation of a feriphys_amd.workloads.SurfaceScene (SpringyMesh::new from the
+    triangle list, then add_strut across every shared edge in strut order)."""
+    from feriphys_amd.workloads import bend_pairs
+
+    dt = np.dtype(dtype).type
+    meshes = []
+    for spec in scene.meshes:
+        m = Mesh.new(spec.positions.astype(dt), spec.indices, spec.mass, spec.k, spec.c, dtype=dt)
+        st = m.struts()
+        _, fv = m.faces()
+        for a, b in bend_pairs(st["v01"], st["faces"], fv):
+            m.add_strut(int(a), int(b), spec.bk, spec.bc)
+        meshes.append(m)
+    obs = [Obstacle(np.asarray(p, np.float32).astype(dt), i, dtype=dt) for p, i in scene.obstacles]
+    sim = Simulation(meshes, obs, dtype=dt)
+    c = scene.config
+    sim.set_config(Config(c.integration, c.dt, c.gravity, c.wind, c.lift_coefficient, c.drag_coefficient,
+                          c.coefficient_of_restitution, c.coefficient_of_friction))
+    return sim
+
+
 def build_scene(scene, dtype=np.float32) -> Simulation:
     """The oracle Simulation of a feriphys_amd.workloads.Scene: the same Cloth::new arguments, the
     same seeded jitter (computed on the f32 records; the f64 run starts from them widened), the
