@@ -40,6 +40,20 @@ class FpTorsionConfig(C.Structure):
     _fields_ = [("spring_constant", C.c_float), ("spring_damping", C.c_float)]
 
 
+class FpSurrogateConfig(C.Structure):
+    _fields_ = [("dt", C.c_float), ("ratio", C.c_int32), ("strain_scale", C.c_float), ("out_scale", C.c_float),
+                ("base_struts", C.c_int32), ("strain_limit", C.c_float), ("energy_limit", C.c_float)]
+
+
+class FpSurrogateStats(C.Structure):
+    _fields_ = [("big_steps", C.c_uint64), ("fallbacks", C.c_uint64), ("mesh_fallbacks", C.c_uint64),
+                ("max_strain_residual", C.c_float), ("max_energy_residual", C.c_float)]
+
+
+FP_SURROGATE_PARAMS = 42755
+FP_SURROGATE_FEATURES = 72
+
+
 SIGNATURES = {
     "fp_last_error": (C.c_char_p, []),
     "fp_version": (C.c_char_p, []),
@@ -89,6 +103,25 @@ SIGNATURES = {
     "fp_sim_time_steps": (C.c_int, [_P, _U64, _FP]),
     "fp_sim_time_steps_graph": (C.c_int, [_P, _U64, C.c_uint32, _FP]),
     "fp_sim_free": (None, [_P]),
+    "fp_surrogate_config_default": (C.c_int, [C.POINTER(FpSurrogateConfig)]),
+    "fp_surrogate_new": (C.c_int, [C.POINTER(FpSurrogateConfig), _U64, _FP, _PP]),
+    "fp_surrogate_get_config": (C.c_int, [_P, C.POINTER(FpSurrogateConfig)]),
+    "fp_surrogate_get_params": (C.c_int, [_P, _FP]),
+    "fp_surrogate_set_params": (C.c_int, [_P, _FP]),
+    "fp_surrogate_grad": (C.c_int, [_P, _FP, _FP, _U64, _FP, _FP, _FP]),
+    "fp_surrogate_reserve": (C.c_int, [_P, _U64]),
+    "fp_surrogate_sample_count": (C.c_int, [_P, _U64P]),
+    "fp_surrogate_clear_samples": (C.c_int, [_P]),
+    "fp_surrogate_add_samples": (C.c_int, [_P, _FP, _FP, _U64]),
+    "fp_surrogate_train": (C.c_int, [_P, _U64, C.c_uint32, C.c_float, _U64, _FP]),
+    "fp_surrogate_free": (None, [_P]),
+    "fp_sim_collect_samples": (C.c_int, [_P, _P, _U64, C.c_float, _U64]),
+    "fp_sim_set_surrogate": (C.c_int, [_P, _P]),
+    "fp_sim_step_surrogate": (C.c_int, [_P, _U64, C.POINTER(FpSurrogateStats)]),
+    "fp_sim_time_surrogate": (C.c_int, [_P, _U64, _FP, _FP, C.POINTER(FpSurrogateStats)]),
+    "fp_sim_surrogate_features": (C.c_int, [_P, _U64, _FP]),
+    "fp_sim_surrogate_accel": (C.c_int, [_P, _U64, _FP]),
+    "fp_sim_surrogate_slots": (C.c_int, [_P, _U64, _I64P, _FP]),
 }
 
 _LIB = None

@@ -1,0 +1,63 @@
+// Host-side helpers shared by the C-ABI translation units: the thread-local error message behind
+// fp_last_error, status plumbing and a minimal owning device buffer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/feriphys_gpu.h"
+#include "mesh_host.hpp"
+
+namespace fph {
+
+inline thread_local std::string g_err;
+
+inline int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+inline int fail(const fp::Status& s) { return fail(s.code, s.msg); }
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fph::fail(FP_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T> struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { reset(); }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    reset();
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)&p, count * sizeof(T));
+  }
+  // Allocates only when the buffer holds fewer than `count` elements.
+  hipError_t reserve(size_t count) {
+    if (p != nullptr && n >= count) return hipSuccess;
+    return alloc(count);
+  }
+  hipError_t upload(const T* h, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
+  }
+  // Like upload, but keeps the allocation (and so the device address) when the size is unchanged.
+  hipError_t assign(const T* h, size_t count) {
+    if (count != n || p == nullptr) return upload(h, count);
+    return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
+  }
+};
+
+}  // namespace fph

@@ -1,0 +1,299 @@
+// C ABI of the learned surrogate network: creation, parameters, GPU training (forward, backward,
+// Adam) and the packing of the inference kernel's weights.
+#include "surrogate_host.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+namespace fph {
+
+namespace {
+
+uint64_t splitmix(uint64_t& x) {
+  uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+uint16_t bf16_bits(float f) {  // round to nearest even (finite inputs)
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+int ensure_rows(fp_surrogate* s, int rows) {
+  if (rows <= s->rows) return FP_OK;
+  const size_t r = (size_t)rows;
+  HIP_TRY(s->Xb.alloc(r * fp::kFeat));
+  HIP_TRY(s->Tb.alloc(r * 3));
+  HIP_TRY(s->Z1.alloc(r * fp::kHid));
+  HIP_TRY(s->Z2.alloc(r * fp::kHid));
+  HIP_TRY(s->Z3.alloc(r * fp::kHid));
+  HIP_TRY(s->H1.alloc(r * fp::kHid));
+  HIP_TRY(s->H2.alloc(r * fp::kHid));
+  HIP_TRY(s->H3.alloc(r * fp::kHid));
+  HIP_TRY(s->Y.alloc(r * 3));
+  HIP_TRY(s->dY.alloc(r * 3));
+  HIP_TRY(s->dH.alloc(r * fp::kHid));
+  HIP_TRY(s->dZ.alloc(r * fp::kHid));
+  HIP_TRY(s->partial.alloc((r * 3 + fp::kBlock - 1) / fp::kBlock));
+  s->rows = rows;
+  return FP_OK;
+}
+
+}  // namespace
+
+int surrogate_fwd_bwd(fp_surrogate* s, int B, int loss_slot) {
+  using namespace fp;
+  hipStream_t st = s->stream;
+  const float* P = s->p.p;
+  float* G = s->g.p;
+  const __bf16* W = s->wb.p;
+  HIP_TRY(launch_to_bf16(P, s->wb.p, kParams, st));
+  // forward: Z_l [B][128] = H_{l-1} W_l^T (+ b_l), H_l = bf16(gelu(Z_l))
+  HIP_TRY(launch_gemm_bf16(B, kHid, kFeat, s->Xb.p, kFeat, 1, W + kOffW1, 1, kFeat, s->Z1.p, kHid, 1, false, st));
+  HIP_TRY(launch_bias_act(s->Z1.p, P + kOffB1, s->H1.p, B, kHid, true, st));
+  HIP_TRY(launch_gemm_bf16(B, kHid, kHid, s->H1.p, kHid, 1, W + kOffW2, 1, kHid, s->Z2.p, kHid, 1, false, st));
+  HIP_TRY(launch_bias_act(s->Z2.p, P + kOffB2, s->H2.p, B, kHid, true, st));
+  HIP_TRY(launch_gemm_bf16(B, kHid, kHid, s->H2.p, kHid, 1, W + kOffW3, 1, kHid, s->Z3.p, kHid, 1, false, st));
+  HIP_TRY(launch_bias_act(s->Z3.p, P + kOffB3, s->H3.p, B, kHid, true, st));
+  HIP_TRY(launch_gemm_bf16(B, kOut, kHid, s->H3.p, kHid, 1, W + kOffW4, 1, kHid, s->Y.p, kOut, 1, false, st));
+  HIP_TRY(launch_mse(s->Y.p, P + kOffB4, s->Tb.p, B, s->dY.p, s->partial.p, st));
+  const int np = (int)((3L * B + kBlock - 1) / kBlock);
+  HIP_TRY(launch_sum(s->partial.p, np, 1.0f / (3.0f * (float)B), s->loss.p + loss_slot, st));
+  // backward
+  // layer 4: dW4 [3][128] = dY^T H3, db4 = sum dY, dH3 [B][128] = dY W4
+  HIP_TRY(launch_gemm_bf16(kOut, kHid, B, s->dY.p, 1, kOut, s->H3.p, kHid, 1, G + kOffW4, kHid, 1, false, st));
+  HIP_TRY(launch_colsum(s->dY.p, B, kOut, G + kOffB4, st));
+  HIP_TRY(launch_gemm_bf16(B, kHid, kOut, s->dY.p, kOut, 1, W + kOffW4, kHid, 1, s->dH.p, kHid, 1, false, st));
+  // layers 3, 2, 1: dZ = bf16(dH * gelu'(Z)), dW = dZ^T H_prev, db = sum dZ, dH_prev = dZ W
+  struct L {
+    const float* Z;
+    const __bf16* Hprev;
+    int kin, offW, offB;
+  } layers[3] = {{s->Z3.p, s->H2.p, kHid, kOffW3, kOffB3},
+                 {s->Z2.p, s->H1.p, kHid, kOffW2, kOffB2},
+                 {s->Z1.p, s->Xb.p, kFeat, kOffW1, kOffB1}};
+  for (int l = 0; l < 3; ++l) {
+    const L& q = layers[l];
+    HIP_TRY(launch_gelu_bwd(s->dH.p, q.Z, s->dZ.p, (long)B * kHid, kHid, st));
+    HIP_TRY(launch_gemm_bf16(kHid, q.kin, B, s->dZ.p, 1, kHid, q.Hprev, q.kin, 1, G + q.offW, q.kin, 1, false, st));
+    HIP_TRY(launch_colsum(s->dZ.p, B, kHid, G + q.offB, st));
+    if (l < 2)
+      HIP_TRY(launch_gemm_bf16(B, kHid, kHid, s->dZ.p, kHid, 1, W + q.offW, kHid, 1, s->dH.p, kHid, 1, false, st));
+  }
+  return FP_OK;
+}
+
+std::vector<uint8_t> pack_image(const std::vector<float>& P) {
+  using namespace fp;
+  std::vector<uint8_t> img(kImgBytes, 0);
+  auto put = [&](int byte_off, int row, int ld, int col, float v) {
+    const uint16_t b = bf16_bits(v);
+    std::memcpy(&img[byte_off + 2 * (row * ld + col)], &b, 2);
+  };
+  // layer 1: K position kb*32 + g*8 + s holds feature (3g + t/6)*6 + t%6 with t = kb*8 + s (t < 18)
+  for (int n = 0; n < kHid; ++n)
+    for (int kb = 0; kb < 3; ++kb)
+      for (int g = 0; g < 4; ++g)
+        for (int s = 0; s < 8; ++s) {
+          const int t = kb * 8 + s;
+          const float w = t < 18 ? P[kOffW1 + n * kFeat + (3 * g + t / 6) * 6 + t % 6] : 0.0f;
+          put(kImgW1, n, kLd1, kb * 32 + g * 8 + s, w);
+        }
+  // layers 2-4: K position kb*32 + g*8 + s holds neuron 32kb + (s < 4 ? 4g + s : 16 + 4g + s - 4)
+  auto perm = [&](int off, int rows, int img_off) {
+    for (int n = 0; n < rows; ++n)
+      for (int kb = 0; kb < 4; ++kb)
+        for (int g = 0; g < 4; ++g)
+          for (int s = 0; s < 8; ++s) {
+            const int k = 32 * kb + (s < 4 ? 4 * g + s : 16 + 4 * g + s - 4);
+            put(img_off, n, kLd, kb * 32 + g * 8 + s, P[off + n * kHid + k]);
+          }
+  };
+  perm(kOffW2, kHid, kImgW2);
+  perm(kOffW3, kHid, kImgW3);
+  perm(kOffW4, kOut, kImgW4);
+  float* b = reinterpret_cast<float*>(&img[kImgB]);
+  for (int j = 0; j < kHid; ++j) {
+    b[j] = P[kOffB1 + j];
+    b[kHid + j] = P[kOffB2 + j];
+    b[2 * kHid + j] = P[kOffB3 + j];
+  }
+  for (int j = 0; j < kOut; ++j) b[3 * kHid + j] = P[kOffB4 + j];
+  return img;
+}
+
+}  // namespace fph
+
+using fph::fail;
+
+extern "C" {
+
+int fp_surrogate_config_default(fp_surrogate_config* out) {
+  if (!out) return fail(FP_EINVAL, "NULL argument");
+  out->dt = 1.0f / 600.0f;
+  out->ratio = 8;
+  out->strain_scale = 10.0f;
+  const float H = out->dt * (float)out->ratio;
+  out->out_scale = 0.005f / (H * H);
+  out->base_struts = 0;
+  out->strain_limit = 0.1f;
+  out->energy_limit = 0.01f;
+  return FP_OK;
+}
+
+int fp_surrogate_new(const fp_surrogate_config* cfg, uint64_t seed, const float* params, fp_surrogate** out) {
+  if (!out) return fail(FP_EINVAL, "out is NULL");
+  *out = nullptr;
+  fp_surrogate_config c;
+  if (cfg) c = *cfg; else fp_surrogate_config_default(&c);
+  if (!(c.dt > 0.0f) || c.ratio < 1 || !(c.out_scale > 0.0f) || !(c.strain_scale > 0.0f))
+    return fail(FP_EINVAL, "bad surrogate config");
+  auto s = std::make_unique<fp_surrogate>();
+  s->cfg = c;
+  HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  std::vector<float> P(fp::kParams);
+  if (params) {
+    std::memcpy(P.data(), params, sizeof(float) * fp::kParams);
+  } else {
+    uint64_t x = seed;
+    auto fill = [&](int off, int count, int fan_in) {
+      const double b = 1.0 / std::sqrt((double)fan_in);
+      for (int i = 0; i < count; ++i) P[off + i] = (float)(((double)(fph::splitmix(x) >> 11) * 0x1.0p-53 * 2.0 - 1.0) * b);
+    };
+    fill(fp::kOffW1, fp::kHid * fp::kFeat + fp::kHid, fp::kFeat);
+    fill(fp::kOffW2, fp::kHid * fp::kHid + fp::kHid, fp::kHid);
+    fill(fp::kOffW3, fp::kHid * fp::kHid + fp::kHid, fp::kHid);
+    fill(fp::kOffW4, fp::kOut * fp::kHid + fp::kOut, fp::kHid);
+  }
+  HIP_TRY(s->p.upload(P.data(), P.size()));
+  HIP_TRY(s->g.alloc(fp::kParams));
+  HIP_TRY(s->m.alloc(fp::kParams));
+  HIP_TRY(s->v.alloc(fp::kParams));
+  HIP_TRY(hipMemset(s->m.p, 0, sizeof(float) * fp::kParams));
+  HIP_TRY(hipMemset(s->v.p, 0, sizeof(float) * fp::kParams));
+  HIP_TRY(s->wb.alloc(fp::kParams));
+  HIP_TRY(s->loss.alloc(1));
+  *out = s.release();
+  return FP_OK;
+}
+
+int fp_surrogate_get_config(const fp_surrogate* s, fp_surrogate_config* out) {
+  if (!s || !out) return fail(FP_EINVAL, "NULL argument");
+  *out = s->cfg;
+  return FP_OK;
+}
+
+int fp_surrogate_get_params(fp_surrogate* s, float* params) {
+  if (!s || !params) return fail(FP_EINVAL, "NULL argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(params, s->p.p, sizeof(float) * fp::kParams, hipMemcpyDeviceToHost));
+  return FP_OK;
+}
+
+int fp_surrogate_set_params(fp_surrogate* s, const float* params) {
+  if (!s || !params) return fail(FP_EINVAL, "NULL argument");
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(s->p.p, params, sizeof(float) * fp::kParams, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(s->m.p, 0, sizeof(float) * fp::kParams));
+  HIP_TRY(hipMemset(s->v.p, 0, sizeof(float) * fp::kParams));
+  s->adam_t = 0;
+  return FP_OK;
+}
+
+int fp_surrogate_grad(fp_surrogate* s, const float* X, const float* T, uint64_t rows, float* loss, float* grad,
+                      float* Y) {
+  if (!s || !X || !T) return fail(FP_EINVAL, "NULL argument");
+  if (rows == 0 || rows > (1u << 20)) return fail(FP_EINVAL, "rows must be in [1, 2^20]");
+  const int B = (int)rows;
+  if (int e = fph::ensure_rows(s, B)) return e;
+  fph::DBuf<float> xf;
+  HIP_TRY(xf.upload(X, rows * fp::kFeat));
+  HIP_TRY(hipMemcpy(s->Tb.p, T, sizeof(float) * 3 * rows, hipMemcpyHostToDevice));
+  HIP_TRY(fp::launch_to_bf16(xf.p, s->Xb.p, (long)rows * fp::kFeat, s->stream));
+  if (int e = fph::surrogate_fwd_bwd(s, B, 0)) return e;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (loss) HIP_TRY(hipMemcpy(loss, s->loss.p, sizeof(float), hipMemcpyDeviceToHost));
+  if (grad) HIP_TRY(hipMemcpy(grad, s->g.p, sizeof(float) * fp::kParams, hipMemcpyDeviceToHost));
+  if (Y) HIP_TRY(hipMemcpy(Y, s->Y.p, sizeof(float) * 3 * rows, hipMemcpyDeviceToHost));
+  return FP_OK;
+}
+
+int fp_surrogate_reserve(fp_surrogate* s, uint64_t capacity) {
+  if (!s) return fail(FP_EINVAL, "NULL argument");
+  if (capacity <= s->X.n / fp::kFeat) return FP_OK;
+  fph::DBuf<__bf16> X2;
+  fph::DBuf<float> T2;
+  HIP_TRY(X2.alloc(capacity * fp::kFeat));
+  HIP_TRY(T2.alloc(capacity * 3));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (s->count) {
+    HIP_TRY(hipMemcpy(X2.p, s->X.p, s->count * fp::kFeat * sizeof(__bf16), hipMemcpyDeviceToDevice));
+    HIP_TRY(hipMemcpy(T2.p, s->T.p, s->count * 3 * sizeof(float), hipMemcpyDeviceToDevice));
+  }
+  std::swap(s->X.p, X2.p);
+  std::swap(s->X.n, X2.n);
+  std::swap(s->T.p, T2.p);
+  std::swap(s->T.n, T2.n);
+  return FP_OK;
+}
+
+int fp_surrogate_sample_count(const fp_surrogate* s, uint64_t* out) {
+  if (!s || !out) return fail(FP_EINVAL, "NULL argument");
+  *out = s->count;
+  return FP_OK;
+}
+
+int fp_surrogate_clear_samples(fp_surrogate* s) {
+  if (!s) return fail(FP_EINVAL, "NULL argument");
+  s->count = 0;
+  return FP_OK;
+}
+
+int fp_surrogate_add_samples(fp_surrogate* s, const float* X, const float* T, uint64_t rows) {
+  if (!s || !X || !T) return fail(FP_EINVAL, "NULL argument");
+  if (rows == 0) return FP_OK;
+  if (int e = fp_surrogate_reserve(s, s->count + rows)) return e;
+  fph::DBuf<float> xf;
+  HIP_TRY(xf.upload(X, rows * fp::kFeat));
+  HIP_TRY(fp::launch_to_bf16(xf.p, s->X.p + s->count * fp::kFeat, (long)rows * fp::kFeat, s->stream));
+  HIP_TRY(hipMemcpyAsync(s->T.p + s->count * 3, T, rows * 3 * sizeof(float), hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  s->count += rows;
+  return FP_OK;
+}
+
+int fp_surrogate_train(fp_surrogate* s, uint64_t steps, uint32_t batch, float lr, uint64_t seed, float* losses) {
+  if (!s) return fail(FP_EINVAL, "NULL argument");
+  if (s->count == 0) return fail(FP_ESTATE, "no training samples");
+  if (batch == 0 || batch > (1u << 20)) return fail(FP_EINVAL, "batch must be in [1, 2^20]");
+  if (!(lr > 0.0f)) return fail(FP_EINVAL, "bad learning rate");
+  const int B = (int)batch;
+  if (int e = fph::ensure_rows(s, B)) return e;
+  fph::DBuf<float> lossv;
+  HIP_TRY(lossv.alloc(steps ? steps : 1));
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (uint64_t k = 0; k < steps; ++k) {
+    uint64_t h = seed ^ (0x51ed27ull * (k + 1));
+    HIP_TRY(fp::launch_gather_batch(s->X.p, s->T.p, s->count, fph::splitmix(h), B, s->Xb.p, s->Tb.p, s->stream));
+    // loss of this batch goes to lossv[k]
+    std::swap(s->loss.p, lossv.p);
+    const int e = fph::surrogate_fwd_bwd(s, B, (int)k);
+    std::swap(s->loss.p, lossv.p);
+    if (e) return e;
+    ++s->adam_t;
+    const float c1 = 1.0f - std::pow(b1, (float)s->adam_t), c2 = 1.0f - std::pow(b2, (float)s->adam_t);
+    HIP_TRY(fp::launch_adam(s->p.p, s->g.p, s->m.p, s->v.p, fp::kParams, lr, b1, b2, eps, c1, c2, s->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  if (losses && steps) HIP_TRY(hipMemcpy(losses, lossv.p, steps * sizeof(float), hipMemcpyDeviceToHost));
+  return FP_OK;
+}
+
+void fp_surrogate_free(fp_surrogate* s) { delete s; }
+
+}  // extern "C"

@@ -330,6 +330,47 @@ class Simulation(_Handle):
         return ms.value
 
 
+    # --- learned surrogate (feriphys_amd.surrogate) ---
+    def set_surrogate(self, net):
+        """Copies the network into the simulation (None detaches)."""
+        check(lib().fp_sim_set_surrogate(self.handle, None if net is None else net.handle))
+
+    def step_surrogate(self, n_big: int = 1):
+        from .surrogate import SurrogateStats
+
+        st = _lib.FpSurrogateStats()
+        check(lib().fp_sim_step_surrogate(self.handle, n_big, C.byref(st)))
+        return SurrogateStats.from_c(st)
+
+    def time_surrogate(self, n_big: int):
+        """(total wall ms, network-kernel ms from HIP events, stats) of n_big surrogate steps."""
+        from .surrogate import SurrogateStats
+
+        st = _lib.FpSurrogateStats()
+        tot, mlp = C.c_float(), C.c_float()
+        check(lib().fp_sim_time_surrogate(self.handle, n_big, C.byref(tot), C.byref(mlp), C.byref(st)))
+        return tot.value, mlp.value, SurrogateStats.from_c(st)
+
+    def surrogate_features(self, mesh: int = 0) -> np.ndarray:
+        n = self.mesh_counts(mesh)[0]
+        out = np.zeros((n, _lib.FP_SURROGATE_FEATURES), np.float32)
+        check(lib().fp_sim_surrogate_features(self.handle, mesh, _fptr(out)))
+        return out
+
+    def surrogate_accel(self, mesh: int = 0) -> np.ndarray:
+        n = self.mesh_counts(mesh)[0]
+        out = np.zeros((n, 3), np.float32)
+        check(lib().fp_sim_surrogate_accel(self.handle, mesh, _fptr(out)))
+        return out
+
+    def surrogate_slots(self, mesh: int = 0):
+        n = self.mesh_counts(mesh)[0]
+        nbr = np.zeros((n, 12), np.int64)
+        rest = np.zeros((n, 12), np.float32)
+        check(lib().fp_sim_surrogate_slots(self.handle, mesh, nbr.ctypes.data_as(C.POINTER(C.c_int64)), _fptr(rest)))
+        return nbr, rest
+
+
 def config_default() -> Config:
     c = _lib.FpConfig()
     lib().fp_config_default(C.byref(c))

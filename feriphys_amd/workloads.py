@@ -225,6 +225,32 @@ def config2(n_cloths: int = 4096, seed: int = 0, smin: int = 8, smax: int = 256,
     return sc
 
 
+def surrogate_training_scene(n_cloths: int = 128, seed: int = 0, smin: int = 12, smax: int = 48) -> Scene:
+    """Cloths whose plain trajectories train the surrogate: rows and cols each log-uniform in
+    [smin, smax], one full edge pinned (chosen at random), config-0 stiffness, mass, spacing,
+    gravity, drag and lift; per-cloth wind with direction uniform on the upper hemisphere and
+    magnitude uniform in [0, 3] m/s; RK4 at dt = 1/600 s; +-1e-3 m jitter.  Corner pins are left
+    out: a cloth hanging from corners diverges in the reference method itself (see config0)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = math.log(smin), math.log(smax + 1)
+    cloths, winds = [], []
+    for k in range(n_cloths):
+        R = int(min(smax, max(smin, math.floor(math.exp(rng.uniform(lo, hi))))))
+        C = int(min(smax, max(smin, math.floor(math.exp(rng.uniform(lo, hi))))))
+        edge = int(rng.integers(0, 4))
+        pins = ([c for c in range(C)] if edge == 0 else [(R - 1) * C + c for c in range(C)] if edge == 1
+                else [r * C for r in range(R)] if edge == 2 else [r * C + C - 1 for r in range(R)])
+        cloths.append(cloth_spec(R, C, pins=pins))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        d[1] = abs(d[1])
+        winds.append(tuple(float(F32(x)) for x in d * rng.uniform(0.0, 3.0)))
+    base = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.01), (0.0, 0.0, 0.0), 0.2, 0.5, 0.95, 0.3).f32()
+    sc = Scene(cloths, base, [], 1e-3, seed, 600, f"{n_cloths} training cloths")
+    sc.mesh_winds = winds
+    return sc
+
+
 def mesh_configs(scene: Scene):
     """The Config of every mesh: the scene's, with the per-cloth wind where the scene has one."""
     winds = getattr(scene, "mesh_winds", None)

@@ -213,6 +213,83 @@ int fp_sim_time_steps(fp_sim* sim, uint64_t n_steps, float* kernel_ms);
 int fp_sim_time_steps_graph(fp_sim* sim, uint64_t n_steps, uint32_t chunk, float* kernel_ms);
 void fp_sim_free(fp_sim* sim);
 
+/* ---------------------------------------------------------------------------------------------
+ * Learned large-step surrogate.  No counterpart in the reference: the reference only steps with
+ * the explicit integrators of state.rs.  A per-vertex network (72 features -> 3 x 128 tanh-GELU ->
+ * 3; bf16 weights and activations, f32 accumulation on the matrix cores) stands in for the strut
+ * forces over a step of H = ratio * dt:
+ *     a  = (gravity + face forces [+ strut forces when base_struts]) / m + out_scale * net(f)
+ *     v' = v + H a,   x' = x + H v'
+ * then the reference's obstacle collision response over H and pinned records kept.  Features of
+ * a vertex, per strut neighbour slot (up to 12; Cloth::new meshes use fixed grid offsets (0,+1),
+ * (0,-1), (+1,0), (-1,0), (-1,+1), (+1,-1), (0,+2), (0,-2), (+2,0), (-2,0), other meshes their
+ * struts in strut order): strain_scale * strain * d/|d| and (v_nbr - v) * H / L.  Velocities of
+ * the surrogate state are position differences over H.  A big step is redone with `ratio` plain
+ * steps for every mesh where a strut strain changed by more than strain_limit, or the energy
+ * (kinetic + spring + gravity, net of the face forces' work) rose by more than energy_limit times
+ * the kinetic + spring energy before and after.
+ * ------------------------------------------------------------------------------------------- */
+typedef struct fp_surrogate fp_surrogate;
+typedef struct fp_surrogate_config {
+  float dt;            /* small step the network replaces `ratio` of (s) */
+  int32_t ratio;       /* big step H = ratio * dt */
+  float strain_scale;  /* position-feature scale */
+  float out_scale;     /* network output unit (m/s^2) */
+  int32_t base_struts; /* 1: the big step also applies the plain strut forces */
+  float strain_limit;  /* fallback threshold on a strut's strain change per big step */
+  float energy_limit;  /* fallback threshold on the relative energy rise per big step */
+} fp_surrogate_config;
+typedef struct fp_surrogate_stats {
+  uint64_t big_steps;      /* big steps taken */
+  uint64_t fallbacks;      /* big steps in which at least one mesh fell back to plain steps */
+  uint64_t mesh_fallbacks; /* (mesh, big step) pairs that fell back */
+  float max_strain_residual;
+  float max_energy_residual;
+} fp_surrogate_stats;
+/* Parameter vector: W1[128][72], b1[128], W2[128][128], b2[128], W3[128][128], b3[128], W4[3][128],
+ * b4[3] (f32, row-major [out][in]). */
+#define FP_SURROGATE_PARAMS 42755
+#define FP_SURROGATE_FEATURES 72
+int fp_surrogate_config_default(fp_surrogate_config* out);
+/* New network, weights drawn uniform in +-1/sqrt(fan_in) from `seed` (params NULL) or copied. */
+int fp_surrogate_new(const fp_surrogate_config* cfg, uint64_t seed, const float* params, fp_surrogate** out);
+int fp_surrogate_get_config(const fp_surrogate* s, fp_surrogate_config* out);
+int fp_surrogate_get_params(fp_surrogate* s, float* params);
+/* Replaces the weights and resets the optimiser state. */
+int fp_surrogate_set_params(fp_surrogate* s, const float* params);
+/* One forward and backward pass on the GPU over `rows` samples (features X[rows][72], targets
+ * T[rows][3] in output units): the mean squared error, its gradient with respect to every
+ * parameter and the outputs Y[rows][3] (any of loss, grad, Y may be NULL). */
+int fp_surrogate_grad(fp_surrogate* s, const float* X, const float* T, uint64_t rows, float* loss, float* grad,
+                      float* Y);
+/* Training set held on the device. */
+int fp_surrogate_reserve(fp_surrogate* s, uint64_t capacity);
+int fp_surrogate_sample_count(const fp_surrogate* s, uint64_t* out);
+int fp_surrogate_clear_samples(fp_surrogate* s);
+int fp_surrogate_add_samples(fp_surrogate* s, const float* X, const float* T, uint64_t rows);
+/* `steps` Adam steps (beta 0.9 / 0.999) on batches of `batch` samples drawn from the training set
+ * by a hash of (seed, step); losses (may be NULL) receives each step's batch loss. */
+int fp_surrogate_train(fp_surrogate* s, uint64_t steps, uint32_t batch, float lr, uint64_t seed, float* losses);
+void fp_surrogate_free(fp_surrogate* s);
+/* Runs n_big * ratio plain steps of the simulation and adds one sample per free vertex and big
+ * step b >= 1 ... n_big to the network's training set: the features of the state at big step b
+ * with positions and position-difference velocities perturbed by normal noise (`noise` m), and
+ * the acceleration that carries it to the plain solver's positions at big step b + 1.  All
+ * meshes must use the network's dt. */
+int fp_sim_collect_samples(fp_sim* sim, fp_surrogate* s, uint64_t n_big, float noise, uint64_t seed);
+/* Copies the network into the simulation (NULL detaches).  All meshes must use its dt. */
+int fp_sim_set_surrogate(fp_sim* sim, const fp_surrogate* s);
+/* n_big surrogate steps (each ratio * dt), with per-mesh fallback to plain steps.  Blocks. */
+int fp_sim_step_surrogate(fp_sim* sim, uint64_t n_big, fp_surrogate_stats* stats);
+/* The same, timed with HIP events on the simulation's stream: total_ms for the whole run,
+ * mlp_ms for the network kernel's launches alone. */
+int fp_sim_time_surrogate(fp_sim* sim, uint64_t n_big, float* total_ms, float* mlp_ms, fp_surrogate_stats* stats);
+/* The network's input features (n x 72) and output accelerations (n x 3, m/s^2) for one mesh at
+ * the current state, and its slot table (n x 12 local neighbour ids, -1 empty; rest lengths). */
+int fp_sim_surrogate_features(fp_sim* sim, uint64_t mesh, float* features);
+int fp_sim_surrogate_accel(fp_sim* sim, uint64_t mesh, float* accel);
+int fp_sim_surrogate_slots(fp_sim* sim, uint64_t mesh, int64_t* nbr, float* rest);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,200 @@
+"""GPU tests of the learned large-step surrogate: the training kernels' loss and gradients against
+the CPU oracle, the feature slots and features, the fused bf16 MFMA inference kernel, the big-step
+update, the fallback to plain steps (whole and per mesh), training on the GPU, and what the
+trained network gains over the 8x step without it."""
+import numpy as np
+import pytest
+
+import feriphys_amd as fa
+import mlp_oracle as mo
+import pyoracle as orc
+from feriphys_amd import workloads as wl
+from gpu_common import announce, bits
+
+pytestmark = pytest.mark.gpu
+
+OFFS = [(0, 1), (0, -1), (1, 0), (-1, 0), (-1, 1), (1, -1), (0, 2), (0, -2), (2, 0), (-2, 0)]
+H = float(np.float32(np.float32(1.0 / 600.0) * np.float32(8.0)))
+
+
+def with_velocities(sim, mesh, scale, seed):
+    rec = sim.get_points(mesh)
+    rng = np.random.default_rng(seed)
+    rec[:, 4:7] = rng.normal(scale=scale, size=(rec.shape[0], 3)).astype(np.float32)
+    sim.set_points(mesh, rec)
+    return rec
+
+
+def test_training_gradient_matches_oracle():
+    net = fa.Surrogate(seed=5)
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(0)
+    X = rng.normal(scale=0.6, size=(1000, 72)).astype(np.float32)
+    T = rng.normal(scale=0.5, size=(1000, 3)).astype(np.float32)
+    loss, g, Y = net.grad(X, T)
+    lo, go, Yo = mo.loss_and_grad(p, X, T, emulate=True)
+    announce(f"surrogate grad: loss gpu {loss:.6f} oracle {lo:.6f}")
+    assert abs(loss - lo) <= 1e-3 * lo
+    assert np.abs(Y - Yo).max() <= 2e-3 * np.abs(Yo).max()
+    for name, a, b in (("W1", mo.OFF_W1, mo.OFF_B1), ("b1", mo.OFF_B1, mo.OFF_W2), ("W2", mo.OFF_W2, mo.OFF_B2),
+                       ("b2", mo.OFF_B2, mo.OFF_W3), ("W3", mo.OFF_W3, mo.OFF_B3), ("b3", mo.OFF_B3, mo.OFF_W4),
+                       ("W4", mo.OFF_W4, mo.OFF_B4), ("b4", mo.OFF_B4, mo.N_PARAMS)):
+        err = np.linalg.norm(g[a:b] - go[a:b]) / np.linalg.norm(go[a:b])
+        announce(f"  {name}: relative gradient error {err:.2e}")
+        assert err < 1e-2, name
+
+
+def test_slots_and_features_match_oracle():
+    sc = wl.config0(steps=1, seed=3, rows=9, cols=13, pins="edge")
+    sim = wl.build_gpu(sc)
+    rec = with_velocities(sim, 0, 0.2, 1)
+    sim.set_surrogate(fa.Surrogate(seed=1))
+    nbr, rest = sim.surrogate_slots(0)
+    C = 13
+    for i in range(nbr.shape[0]):
+        r, c = divmod(i, C)
+        for j, (dr, dc) in enumerate(OFFS):
+            if nbr[i, j] >= 0:
+                assert nbr[i, j] == (r + dr) * C + (c + dc)
+    assert (nbr[:, 10:] < 0).all()
+    assert (nbr >= 0).sum() == 2 * sim.mesh_counts(0)[1]  # every strut seen from both ends
+    f = sim.surrogate_features(0)
+    fo = mo.features(rec[:, 1:4], rec[:, 4:7], nbr, rest, H, 10.0)
+    np.testing.assert_allclose(f, fo, rtol=2e-5, atol=2e-6)
+
+
+def test_inference_kernel_matches_oracle():
+    """Two meshes (2,000 and 561 vertices: several 512-vertex tiles, one partial)."""
+    a = wl.config0(steps=1, seed=4, rows=40, cols=50, pins="edge")
+    b = wl.config0(steps=1, seed=5, rows=17, cols=33, pins="edge")
+    sc = wl.Scene(a.cloths + b.cloths, a.config, [], 1e-3, 4, 1, "two cloths")
+    sim = wl.build_gpu(sc)
+    net = fa.Surrogate(seed=7)
+    p = net.get_params().astype(np.float64)
+    sim.set_surrogate(net)
+    out_scale = net.config.out_scale
+    for m in range(2):
+        rec = with_velocities(sim, m, 0.3, 10 + m)
+        nbr, rest = sim.surrogate_slots(m)
+        fo = mo.features(rec[:, 1:4], rec[:, 4:7], nbr, rest, H, 10.0)
+        yo, _ = mo.forward(p, fo, emulate=True)
+        got = sim.surrogate_accel(m)
+        err = np.abs(got - yo * out_scale).max() / np.abs(yo * out_scale).max()
+        announce(f"inference mesh {m}: {rec.shape[0]} vertices, max error {err:.2e} of the largest output")
+        assert err < 1e-2
+
+
+def test_big_step_update():
+    """One big step with the fallback off: a = (gravity + face forces) / m + net, v' = v + H a,
+    x' = x + H v'; pinned records kept."""
+    sc = wl.config0(steps=1, seed=6, rows=8, cols=11, pins="edge")
+    sim = wl.build_gpu(sc)
+    rec = with_velocities(sim, 0, 0.2, 2)
+    net = fa.Surrogate(fa.SurrogateConfig(strain_limit=1e30, energy_limit=1e30), seed=3)
+    sim.set_surrogate(net)
+    da = sim.surrogate_accel(0).astype(np.float64)
+    c = sc.config
+    ocfg = orc.Config(c.integration, c.dt, c.gravity, c.wind, c.lift_coefficient, c.drag_coefficient,
+                      c.coefficient_of_restitution, c.coefficient_of_friction)
+    om = orc.Mesh.cloth(*sc.cloths[0].args(), plane=sc.cloths[0].plane)
+    om.set_points(rec)
+    om.clear_forces()
+    om.apply_face_forces(ocfg)
+    F = om.points()[:, 7:10].astype(np.float64) + np.array(c.gravity)
+    m = rec[:, 0].astype(np.float64)
+    a = F / m[:, None] + da
+    v1 = rec[:, 4:7] + H * a
+    x1 = rec[:, 1:4] + H * v1
+    pins = sc.cloths[0].pins
+    x1[pins], v1[pins] = rec[pins, 1:4], rec[pins, 4:7]
+    st = sim.step_surrogate(1)
+    assert st.fallbacks == 0 and st.big_steps == 1
+    got = sim.get_points(0)
+    np.testing.assert_allclose(got[:, 1:4], x1, rtol=0, atol=2e-6)
+    np.testing.assert_allclose(got[:, 4:7], v1, rtol=0, atol=2e-4)
+    assert (bits(got[pins]) == bits(rec[pins])).all()
+
+
+def test_fallback_everywhere_is_the_plain_solver():
+    sc = wl.config0(steps=1, seed=7, rows=12, cols=10, pins="edge")
+    a, b = wl.build_gpu(sc), wl.build_gpu(sc)
+    a.set_surrogate(fa.Surrogate(fa.SurrogateConfig(strain_limit=-1.0), seed=1))
+    st = a.step_surrogate(3)
+    assert st.big_steps == 3 and st.fallbacks == 3 and st.mesh_fallbacks == 3
+    b.step(24)
+    assert (bits(a.get_points(0)) == bits(b.get_points(0))).all()
+
+
+def test_fallback_is_per_mesh():
+    """A free cloth in uniform free fall passes (no strut changes length); in a horizontal cloth
+    pinned at one corner the pin's neighbours fall by H^2 g, stretching their struts by ~6e-4, past
+    the limit, so that mesh alone is redone with plain steps."""
+    free = wl.cloth_spec(6, 7, pins=[])
+    held = wl.cloth_spec(7, 5, pins=[0])
+    cfg = wl.ConfigSpec(1, 1.0 / 600.0, wl.gravity_force(0.01), (0.0, 0.0, 0.0), 0.2, 0.5, 0.95, 0.3).f32()
+    both = wl.Scene([free, held], cfg, [], 0.0, 0, 1, "two")
+    net_cfg = fa.SurrogateConfig(strain_limit=1e-4, energy_limit=1e30)
+    zero = np.zeros(fa.surrogate.N_PARAMS, np.float32)
+    sim = wl.build_gpu(both)
+    sim.set_surrogate(fa.Surrogate(net_cfg, params=zero))
+    st = sim.step_surrogate(1)
+    announce(f"per-mesh fallback: {st}")
+    assert st.fallbacks == 1 and st.mesh_fallbacks == 1
+    only_free = wl.build_gpu(wl.Scene([free], cfg, [], 0.0, 0, 1, "free"))
+    only_free.set_surrogate(fa.Surrogate(net_cfg, params=zero))
+    assert only_free.step_surrogate(1).fallbacks == 0
+    only_held = wl.build_gpu(wl.Scene([held], cfg, [], 0.0, 0, 1, "held"))
+    only_held.step(8)
+    assert (bits(sim.get_points(0)) == bits(only_free.get_points(0))).all()
+    assert (bits(sim.get_points(1)) == bits(only_held.get_points(0))).all()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    """A network trained on the GPU from 96 training cloths (75 big steps each)."""
+    sc = wl.surrogate_training_scene(n_cloths=96, seed=11, smin=12, smax=40)
+    sim = wl.build_gpu(sc)
+    net = fa.Surrogate(seed=0)
+    net.collect(sim, n_big=75, noise=1e-3, seed=1)
+    n = net.sample_count()
+    free = sum(c.rows * c.cols - len(c.pins) for c in sc.cloths)
+    assert n == 75 * free
+    losses = net.train(1500, batch=8192, lr=1e-3, seed=2)
+    losses2 = net.train(1000, batch=8192, lr=3e-4, seed=3)
+    announce(f"trained on {n} samples: loss {losses[:5].mean():.4f} -> {losses[-50:].mean():.4f} -> "
+             f"{losses2[-50:].mean():.4f}")
+    return net, losses, losses2
+
+
+def test_training_reduces_loss(trained):
+    _, losses, losses2 = trained
+    assert np.isfinite(losses).all() and np.isfinite(losses2).all()
+    assert losses2[-50:].mean() < 0.25 * losses[:5].mean()
+
+
+def test_trained_network_gains_over_the_big_step_without_it(trained):
+    net, _, _ = trained
+    sc = wl.config0(steps=1, seed=21, rows=32, cols=32, pins="edge")
+    ref = wl.build_gpu(sc)
+    sur = wl.build_gpu(sc)
+    big = wl.build_gpu(sc)
+    c = sc.config
+    big.config = fa.Config(fa.Integration(c.integration), H, c.gravity, c.wind, c.lift_coefficient,
+                           c.drag_coefficient, c.coefficient_of_restitution, c.coefficient_of_friction)
+    off = fa.Surrogate(fa.SurrogateConfig(strain_limit=1e30, energy_limit=1e30), params=net.get_params())
+    sur.set_surrogate(off)
+    extent = 31 * 0.05
+    out = []
+    for k in range(1, 11):
+        ref.step(8)
+        sur.step_surrogate(1)
+        big.step(1)
+        x = ref.get_points(0)[:, 1:4].astype(np.float64)
+        es = np.sqrt(np.mean(np.sum((sur.get_points(0)[:, 1:4] - x) ** 2, axis=1))) / extent
+        eb = np.sqrt(np.mean(np.sum((big.get_points(0)[:, 1:4] - x) ** 2, axis=1))) / extent
+        out.append((k, es, eb))
+    for k, es, eb in out:
+        announce(f"big step {k}: rms position error / extent: surrogate {es:.2e}, 8x step without it {eb:.2e}")
+    es, eb = out[-1][1], out[-1][2]
+    assert np.isfinite(es) and es < 5e-2
+    assert not np.isfinite(eb) or eb > 10 * es

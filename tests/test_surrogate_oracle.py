@@ -1,0 +1,71 @@
+"""Pins the surrogate network's CPU oracle (oracle/mlp_oracle.py) before it judges the GPU path:
+the analytic gradient against central finite differences, the activation against the exact
+(erf) GELU it approximates, hand-computed features, and the bf16 rounding rule."""
+import numpy as np
+import pytest
+from scipy.special import erf
+
+import mlp_oracle as mo
+
+
+def test_bf16_rounding_ties_to_even():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1.0 + 2 ** -7 + 2 ** -9, 3.0e38])
+    want = np.array([1.0, 1.0, 1.0 + 2 ** -6, -2.5, 1.0 + 2 ** -7, 1.765625 * 2.0 ** 127])  # 3e38: 0x7f61b1e6 -> 0x7f62
+    got = mo.bf16(x)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_gelu_tracks_exact_gelu_and_its_derivative():
+    z = np.linspace(-8, 8, 4001)
+    exact = 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
+    assert np.max(np.abs(mo.gelu(z) - exact)) < 5e-4  # published bound of the tanh form (~4.7e-4)
+    assert mo.gelu(0.0) == 0.0
+    assert abs(mo.gelu(20.0) - 20.0) < 1e-12 and abs(mo.gelu(-20.0)) < 1e-12
+    h = 1e-6
+    fd = (mo.gelu(z + h) - mo.gelu(z - h)) / (2 * h)
+    np.testing.assert_allclose(mo.gelu_grad(z), fd, rtol=0, atol=1e-8)
+
+
+def test_features_hand_case():
+    pos = np.array([[0.0, 0.0, 0.0], [0.06, 0.0, 0.0], [0.0, 0.0, 0.04]])
+    vel = np.array([[0.0, 0.0, 0.0], [0.0, 0.3, 0.0], [0.0, 0.0, 0.0]])
+    nbr = -np.ones((3, 12), np.int64)
+    rest = np.zeros((3, 12))
+    nbr[0, 0], rest[0, 0] = 1, 0.05  # stretched by 20 %
+    nbr[0, 5], rest[0, 5] = 2, 0.05  # compressed by 20 %
+    H = 1.0 / 75.0
+    f = mo.features(pos, vel, nbr, rest, H, 10.0).reshape(3, 12, 6)
+    np.testing.assert_allclose(f[0, 0], [2.0, 0.0, 0.0, 0.0, 0.3 * H / 0.05, 0.0], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(f[0, 5], [0.0, 0.0, -2.0, 0.0, 0.0, 0.0], rtol=1e-12, atol=1e-15)
+    assert not f[0, [1, 2, 3, 4, 6, 7, 8, 9, 10, 11]].any() and not f[1:].any()
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(0)
+    p = mo.init_params(1)
+    X = rng.normal(scale=0.7, size=(6, 72))
+    T = rng.normal(scale=0.5, size=(6, 3))
+    loss, g, _ = mo.loss_and_grad(p, X, T)
+    # every bias, and a spread of weights of every layer
+    idx = np.concatenate([np.arange(mo.OFF_B1, mo.OFF_W2), np.arange(mo.OFF_B4, mo.N_PARAMS),
+                          rng.choice(np.arange(mo.OFF_W1, mo.OFF_B1), 40, replace=False),
+                          rng.choice(np.arange(mo.OFF_W2, mo.OFF_B2), 40, replace=False),
+                          rng.choice(np.arange(mo.OFF_W3, mo.OFF_B3), 40, replace=False),
+                          rng.choice(np.arange(mo.OFF_W4, mo.OFF_B4), 40, replace=False)])
+    h = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (mo.loss_and_grad(pp, X, T)[0] - mo.loss_and_grad(pm, X, T)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+@pytest.mark.parametrize("emulate", [False, True])
+def test_loss_is_mean_squared_error(emulate):
+    rng = np.random.default_rng(2)
+    p = mo.init_params(3)
+    X = rng.normal(size=(5, 72))
+    T = rng.normal(size=(5, 3))
+    loss, _, Y = mo.loss_and_grad(p, X, T, emulate=emulate)
+    assert loss == pytest.approx(float(np.mean((Y - T) ** 2)), rel=1e-12)
