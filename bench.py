@@ -41,6 +41,8 @@ def main():
                     help="cloth: one 1024x1024 cloth + sphere (default); batched: 4096 cloths of uneven sizes; "
                          "surfaces: 64 perturbed icospheres (2.1M vertices) on a ground plane")
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph (batched)")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "feriphys_amd", "weights", "cloth_surrogate.npy"),
+                    help="surrogate network for the cloth workload's surrogate leg ('' to skip it)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -88,6 +90,8 @@ def main():
     kernel_avg_s = kernel_ms * 1e-3 / args.steps
     achieved = BYTES_PER_VERTEX_STEP * N / kernel_avg_s / 1e9
 
+    sur = surrogate_leg(args, np, fa, wl, kern, pts, initial, N, dt, sim_s_per_s)
+
     cpu = None
     if args.cpu_steps > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -109,9 +113,13 @@ def main():
             "vertex_steps_per_s": args.cpu_steps * N / cpu_wall,
         }
 
+    # the headline is the fastest form that meets the error bar (the plain step has no error)
+    use_sur = sur is not None and sur["rms_error_over_extent"] <= 1e-3
     line = {
         "metric": "sim-s/s, 1M-vert cloth: surrogate vs plain @<=1e-3 pos err; %HBM peak",
-        "value": sim_s_per_s,
+        "value": sur["sim_s_per_s"] if use_sur else sim_s_per_s,
+        "value_source": "surrogate" if use_sur else "plain",
+        "plain_sim_s_per_s": sim_s_per_s,
         "unit": "sim-s/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -126,13 +134,62 @@ def main():
                                f"320-face icosphere collider, RK4 dt=1/600 s",
                    "vertices": N, "dt": dt, "kernel": args.kernel, "math": args.math},
         "vertex_steps_per_s": args.steps * N / wall,
-        "surrogate": None,
+        "surrogate": sur,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": BYTES_PER_VERTEX_STEP * N},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+MLP_FLOP_PER_VERTEX = 2 * (72 * 128 + 128 * 128 + 128 * 128 + 128 * 3)  # 84,736
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def surrogate_leg(args, np, fa, wl, kern, ref, initial, N, dt, plain_sim_s_per_s):
+    """The same cloth stepped by the learned surrogate (8x steps, per-mesh fallback on) for the same
+    simulated time, timed on the host clock around the whole run (it reads each big step's
+    diagnostics back); the network kernel's launches timed with HIP events.  Error: the final
+    positions against the plain run's."""
+    if not args.weights or not os.path.exists(args.weights):
+        return None
+    params = np.load(args.weights, allow_pickle=False)
+    net = fa.Surrogate(fa.SurrogateConfig(), params=params)
+    ratio = net.config.ratio
+    n_big = args.steps // ratio
+    sim = wl.build_gpu(wl.config1(n=args.n, steps=args.steps))
+    sim.set_kernel(kern, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
+    sim.set_surrogate(net)
+    if args.warmup:
+        sim.step_surrogate(max(1, args.warmup // ratio))
+        sim.set_points(0, initial)
+    sim.synchronize()
+    total_ms, mlp_ms, st = sim.time_surrogate(n_big)
+    x = sim.get_points(0)[:, 1:4].astype(np.float64)
+    xr = ref[:, 1:4].astype(np.float64)
+    x0 = initial[:, 1:4].astype(np.float64)
+    extent = (args.n - 1) * 0.05
+    e = np.linalg.norm(x - xr, axis=1)
+    disp = float(np.sqrt(np.mean(np.sum((xr - x0) ** 2, axis=1))))
+    sim_s = n_big * ratio * dt / (total_ms * 1e-3)
+    mlp_avg_s = mlp_ms * 1e-3 / max(1, n_big)
+    tflops = MLP_FLOP_PER_VERTEX * N / mlp_avg_s / 1e12 if mlp_avg_s > 0 else None
+    return {
+        "sim_s_per_s": sim_s, "speedup_vs_plain": sim_s / plain_sim_s_per_s, "big_steps": n_big, "ratio": ratio,
+        "ms_per_big_step": total_ms / max(1, n_big),
+        "rms_error_over_extent": float(np.sqrt(np.mean(e * e))) / extent,
+        "max_error_over_extent": float(e.max()) / extent,
+        "rms_error_over_rms_displacement": float(np.sqrt(np.mean(e * e))) / disp if disp > 0 else None,
+        "finite": bool(np.isfinite(x).all()),
+        "fallbacks": st.fallbacks, "max_strain_residual": st.max_strain_residual,
+        "max_energy_residual": st.max_energy_residual,
+        "error_definition": "final positions vs the plain dt=1/600 run after the same simulated time; "
+                            f"extent = cloth width {extent:.2f} m",
+        "mlp_roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tflops / MFMA_BF16_PEAK_TFLOPS if tflops else None,
+                         "kernel_avg_us": mlp_avg_s * 1e6, "flop_per_launch": MLP_FLOP_PER_VERTEX * N},
+    }
 
 
 def bench_batched(args, np, fa, wl):

@@ -1336,10 +1336,48 @@ int fp_sim_collect_samples(fp_sim* s, fp_surrogate* n, uint64_t n_big, float noi
   if (int e = snap(2)) return e;
   const fp::FeatParams fpp = feat_params(n->cfg);
   const uint32_t nfb = (uint32_t)s->h_face_blocks.size();
+  // Trajectory health: a mesh whose plain state stops being finite, or reaches a speed of 30 m/s or
+  // a strain of 100 %, contributes no samples from that big step on (the reference method itself
+  // diverges for some pinnings, see the cloth workloads).
+  const size_t M = s->meshes.size();
+  const uint32_t nblk = (uint32_t)s->h_blocks.size();
+  std::vector<uint8_t> sick(M, 0);
+  std::vector<float> health(nblk);
+  DBuf<float> d_health;
+  DBuf<uint32_t> ids_now;
+  HIP_TRY(d_health.alloc(nblk + 1));
+  uint64_t n_ids = 0;
+  auto rebuild_ids = [&]() -> int {
+    std::vector<uint32_t> ids;
+    for (size_t m = 0; m < M; ++m) {
+      if (sick[m]) continue;
+      const MeshSlot& ms = s->meshes[m];
+      std::vector<uint8_t> pin(ms.host.n, 0);
+      for (uint64_t q : ms.host.pins) pin[q] = 1;
+      for (uint64_t i = 0; i < ms.host.n; ++i)
+        if (!pin[i]) ids.push_back((uint32_t)(ms.vbase + i));
+    }
+    n_ids = ids.size();
+    HIP_TRY(ids_now.upload(ids.data(), ids.size()));
+    return FP_OK;
+  };
+  if (int e = rebuild_ids()) return e;
   for (uint64_t b = 0; b < n_big; ++b) {
     if (int e = run_steps(s, (uint64_t)n->cfg.ratio)) return e;
     const int ic = (int)(b % 3), in = (int)((b + 1) % 3), ip = (int)((b + 2) % 3);
     if (int e = snap(in)) return e;
+    fp::StepArgs st_now = s->args(false);
+    HIP_TRY(fp::launch_sur_health(st_now.in, S.slots.p, s->all_blocks.p, nblk, 30.0f, 1.0f, d_health.p, s->stream));
+    HIP_TRY(hipMemcpyAsync(health.data(), d_health.p, nblk * sizeof(float), hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    bool changed = false;
+    for (uint32_t k = 0; k < nblk; ++k) {
+      const uint32_t m = s->h_blocks[k].mesh;
+      if (!sick[m] && !(health[k] <= 1.0f)) { sick[m] = 1; changed = true; }
+    }
+    if (changed)
+      if (int e = rebuild_ids()) return e;
+    if (n_ids == 0) break;
     const fp::Pos3 prev{pos[ip][0].p, pos[ip][1].p, pos[ip][2].p}, cur{pos[ic][0].p, pos[ic][1].p, pos[ic][2].p},
         next{pos[in][0].p, pos[in][1].p, pos[in][2].p};
     fp::SoA noisy{nz[0].p, nz[1].p, nz[2].p, nz[3].p, nz[4].p, nz[5].p};
@@ -1349,10 +1387,10 @@ int fp_sim_collect_samples(fp_sim* s, fp_surrogate* n, uint64_t n_big, float noi
     a.in = noisy;
     a.params = S.big_params.p;
     HIP_TRY(fp::launch_faces(a, s->csr(), s->all_face_blocks.p, nfb, false, s->stream));
-    HIP_TRY(fp::launch_sur_samples(a, s->csr(), S.ids.p, S.mesh_of.p, (uint32_t)nfree, next, S.slots.p, fpp,
+    HIP_TRY(fp::launch_sur_samples(a, s->csr(), ids_now.p, S.mesh_of.p, (uint32_t)n_ids, next, S.slots.p, fpp,
                                    n->cfg.out_scale, n->cfg.base_struts != 0, n->X.p + n->count * fp::kFeat,
                                    n->T.p + n->count * 3, s->stream));
-    n->count += nfree;
+    n->count += n_ids;
   }
   HIP_TRY(hipStreamSynchronize(s->stream));
   if (S.on) return upload_big_params(s, S.cfg);

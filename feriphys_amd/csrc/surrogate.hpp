@@ -2,8 +2,9 @@
 // and the launchers of surrogate_kernels.hip.
 //
 // The network maps 72 features of a vertex (its up-to-12 strut neighbours, 6 values each) through
-// three hidden layers of 128 tanh-GELU units to a 3-vector, the correction acceleration for a step
-// of H = ratio * dt.  Weights and activations are bf16, products accumulate in f32 (MFMA).
+// three hidden layers of 128 GELU units (a polynomial GELU, see surrogate_kernels.hip) to a
+// 3-vector, the correction acceleration for a step of H = ratio * dt.  Weights and activations
+// are bf16, products accumulate in f32 (MFMA).
 #pragma once
 
 #include "kernels.hpp"
@@ -67,6 +68,10 @@ hipError_t launch_sur_integrate(const StepArgs& a, const CsrView& t, const Block
 hipError_t launch_sur_diag(const SoA& in, const SoA& out, const SlotEnt* slots, const float* slot_k,
                            const StepParams* params, const float* mass, const float* work, const BlockRange* blocks,
                            uint32_t n_blocks, DiagOut* diag, hipStream_t s);
+// Per block of `blocks`: max over its vertices of |v| / vmax and |strain| / smax (+inf when any
+// value is not finite).
+hipError_t launch_sur_health(const SoA& st, const SlotEnt* slots, const BlockRange* blocks, uint32_t n_blocks,
+                             float vmax, float smax, float* out, hipStream_t s);
 // Features (natural order: slot * 6 + component, f32) of the vertices `ids`.
 hipError_t launch_sur_features(const SoA& st, const SlotEnt* slots, const uint32_t* ids, uint32_t n, FeatParams fp,
                                float* out, hipStream_t s);

@@ -20,16 +20,24 @@ __device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// tanh-form GELU: 0.5 x (1 + tanh(sqrt(2/pi) (x + 0.044715 x^3))), tanh(y) = 1 - 2 / (exp(2y) + 1)
-__device__ __forceinline__ float gelu_t(float x) {
-  const float y = 0.7978845608028654f * (x + 0.044715f * (x * x * x));
-  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__builtin_amdgcn_exp2f(2.8853900817779268f * y) + 1.0f);
+// The network's activation, a GELU without transcendentals (within 1.3e-3 of x Phi(x) everywhere):
+//   act(x) = x * clamp(0.5 + x P(x^2), 0, 1),  P(u) = c0 + c1 u + c2 u^2 + c3 u^3 + c4 u^4
+// (the clamp engages at |x| >= 3.361, where act is exactly x or 0).  Nine VALU operations per
+// element instead of an exp and a reciprocal.
+constexpr float kG0 = 0.39661726355552673f, kG1 = -0.06160679832100868f, kG2 = 0.0071685598231852055f,
+                kG3 = -0.0004622247943188995f, kG4 = 1.2258552487764973e-05f;
+__device__ __forceinline__ float act_poly(float u) { return fmaf(fmaf(fmaf(fmaf(kG4, u, kG3), u, kG2), u, kG1), u, kG0); }
+__device__ __forceinline__ float gelu(float x) {
+  const float s = __builtin_amdgcn_fmed3f(fmaf(x, act_poly(x * x), 0.5f), 0.0f, 1.0f);
+  return x * s;
 }
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + gelu_t(x)); }
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float t = gelu_t(x);
-  const float dy = 0.7978845608028654f * (1.0f + 0.134145f * (x * x));
-  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * dy;
+  const float u = x * x;
+  const float su = fmaf(x, act_poly(u), 0.5f);
+  const float s = __builtin_amdgcn_fmed3f(su, 0.0f, 1.0f);
+  const float dp = fmaf(fmaf(fmaf(4.0f * kG4, u, 3.0f * kG3), u, 2.0f * kG2), u, kG1);  // P'(u)
+  const float ds = (su > 0.0f && su < 1.0f) ? fmaf(2.0f * u, dp, act_poly(u)) : 0.0f;  // d s / d x
+  return fmaf(x, ds, s);
 }
 
 __device__ __forceinline__ void store_state(const SoA& o, uint32_t i, f3 x, f3 v) {
@@ -71,13 +79,16 @@ __device__ __forceinline__ float unit_open(uint64_t h) { return ((float)(h >> 40
 constexpr int kMlpThreads = 512;
 constexpr int kMlpTile = 512;
 
+// acc = W B + bias (the bias is the accumulators' initial value)
 template <int KB, int LDW>
-__device__ __forceinline__ void mfma_layer(const __bf16* __restrict__ W, const bf16x8 (&B)[KB][4], f32x4 (&acc)[8][4],
-                                           int col, int g) {
+__device__ __forceinline__ void mfma_layer(const __bf16* __restrict__ W, const float* __restrict__ bias,
+                                           const bf16x8 (&B)[KB][4], f32x4 (&acc)[8][4], int col, int g) {
 #pragma unroll
-  for (int mt = 0; mt < 8; ++mt)
+  for (int mt = 0; mt < 8; ++mt) {
+    const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = b;
+  }
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
@@ -88,18 +99,13 @@ __device__ __forceinline__ void mfma_layer(const __bf16* __restrict__ W, const b
     }
 }
 
-__device__ __forceinline__ void bias_gelu_pack(const f32x4 (&acc)[8][4], const float* __restrict__ bias,
-                                               bf16x8 (&out)[4][4], int g) {
+__device__ __forceinline__ void gelu_pack(const f32x4 (&acc)[8][4], bf16x8 (&out)[4][4]) {
 #pragma unroll
-  for (int mt = 0; mt < 8; ++mt) {
-    float b[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) b[r] = bias[16 * mt + 4 * g + r];
+  for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) out[mt >> 1][nt][(mt & 1) * 4 + r] = (__bf16)gelu(acc[mt][nt][r] + b[r]);
-  }
+      for (int r = 0; r < 4; ++r) out[mt >> 1][nt][(mt & 1) * 4 + r] = (__bf16)gelu(acc[mt][nt][r]);
 }
 
 __global__ void __launch_bounds__(kMlpThreads) sur_mlp_kernel(SoA S, const SlotEnt* __restrict__ slots, uint32_t n,
@@ -140,12 +146,12 @@ __global__ void __launch_bounds__(kMlpThreads) sur_mlp_kernel(SoA S, const SlotE
     }
     f32x4 acc[8][4];
     bf16x8 Bh[4][4];
-    mfma_layer<3, kLd1>(W1, B1, acc, col, g);
-    bias_gelu_pack(acc, Bias, Bh, g);
-    mfma_layer<4, kLd>(W2, Bh, acc, col, g);
-    bias_gelu_pack(acc, Bias + kHid, Bh, g);
-    mfma_layer<4, kLd>(W3, Bh, acc, col, g);
-    bias_gelu_pack(acc, Bias + 2 * kHid, Bh, g);
+    mfma_layer<3, kLd1>(W1, Bias, B1, acc, col, g);
+    gelu_pack(acc, Bh);
+    mfma_layer<4, kLd>(W2, Bias + kHid, Bh, acc, col, g);
+    gelu_pack(acc, Bh);
+    mfma_layer<4, kLd>(W3, Bias + 2 * kHid, Bh, acc, col, g);
+    gelu_pack(acc, Bh);
     f32x4 o[4];
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) o[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -266,6 +272,35 @@ __global__ void __launch_bounds__(kBlock) sur_diag_kernel(SoA I, SoA O, const Sl
   if (threadIdx.x == 0) out[blockIdx.x] = DiagOut{s_ds[0], s_eo[0], s_en[0], s_em[0]};
 }
 
+// Health of the plain trajectories that feed training: per block, the largest of |v| / vmax and
+// strain / smax over its vertices (NaN and infinities count as unhealthy: the result is then +inf).
+__global__ void __launch_bounds__(kBlock) sur_health_kernel(SoA S, const SlotEnt* __restrict__ slots,
+                                                             const BlockRange* __restrict__ blocks, float vmax,
+                                                             float smax, float* __restrict__ out) {
+  __shared__ float s_h[kBlock];
+  const BlockRange br = blocks[blockIdx.x];
+  const uint32_t i = br.begin + threadIdx.x;
+  float h = 0.0f;
+  if (i < br.end) {
+    const f3 p = ld3(S.x, S.y, S.z, i), v = ld3(S.vx, S.vy, S.vz, i);
+    h = __builtin_sqrtf(dot(v, v)) / vmax;
+    for (int j = 0; j < kSlots; ++j) {
+      const SlotEnt e = slots[(size_t)i * kSlots + j];
+      if (e.nbr == kNoSlot) continue;
+      const f3 d = sub(ld3(S.x, S.y, S.z, e.nbr), p);
+      h = fmaxf(h, __builtin_fabsf(__builtin_sqrtf(dot(d, d)) - e.rest) / (e.rest * smax));
+    }
+    if (!(h <= 3.0e38f)) h = __builtin_inff();
+  }
+  s_h[threadIdx.x] = h;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s_h[threadIdx.x] = fmaxf(s_h[threadIdx.x], s_h[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = s_h[0];
+}
+
 __global__ void __launch_bounds__(kBlock) sur_features_kernel(SoA S, const SlotEnt* __restrict__ slots,
                                                                const uint32_t* __restrict__ ids, uint32_t n,
                                                                FeatParams fp, float* __restrict__ out) {
@@ -308,11 +343,10 @@ __global__ void __launch_bounds__(kBlock) sur_samples_kernel(StepArgs A, CsrView
   const StepParams P = A.params[mesh_of[i]];
   const SoA& S = A.in;
   const f3 p = ld3(S.x, S.y, S.z, i), v = ld3(S.vx, S.vy, S.vz, i);
-  for (int s = 0; s < kSlots; ++s) {
-    float f[6];
-    slot_feat(S, p, v, slots[(size_t)i * kSlots + s], fp, f);
-    for (int c = 0; c < 6; ++c) X[(size_t)j * kFeat + 6 * s + c] = (__bf16)f[c];
-  }
+  float f[kFeat];
+  bool ok = true;
+  for (int s = 0; s < kSlots; ++s) slot_feat(S, p, v, slots[(size_t)i * kSlots + s], fp, f + 6 * s);
+  for (int c = 0; c < kFeat; ++c) ok = ok && __builtin_isfinite(f[c]);
   const float m = P.uniform_mass ? P.mass : A.mass[i];
   f3 F = P.gravity;
   for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {
@@ -327,9 +361,12 @@ __global__ void __launch_bounds__(kBlock) sur_samples_kernel(StepArgs A, CsrView
       strut_accumulate<FastMath>(F, p, v, ld3(S.x, S.y, S.z, se.nbr), ld3(S.vx, S.vy, S.vz, se.nbr), se.k, se.c, se.len);
     }
   const float H = fp.H, iH = 1.0f / H, iH2 = iH * iH, im = 1.0f / m, is = 1.0f / out_scale;
-  Tg[3 * (size_t)j] = ((nx.x[i] - p.x) * iH2 - v.x * iH - F.x * im) * is;
-  Tg[3 * (size_t)j + 1] = ((nx.y[i] - p.y) * iH2 - v.y * iH - F.y * im) * is;
-  Tg[3 * (size_t)j + 2] = ((nx.z[i] - p.z) * iH2 - v.z * iH - F.z * im) * is;
+  float t[3] = {((nx.x[i] - p.x) * iH2 - v.x * iH - F.x * im) * is, ((nx.y[i] - p.y) * iH2 - v.y * iH - F.y * im) * is,
+                ((nx.z[i] - p.z) * iH2 - v.z * iH - F.z * im) * is};
+  for (int c = 0; c < 3; ++c) ok = ok && __builtin_isfinite(t[c]);
+  // a non-finite sample (from a trajectory the health check has not yet dropped) becomes a zero row
+  for (int c = 0; c < kFeat; ++c) X[(size_t)j * kFeat + c] = (__bf16)(ok ? f[c] : 0.0f);
+  for (int c = 0; c < 3; ++c) Tg[3 * (size_t)j + c] = ok ? t[c] : 0.0f;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -505,6 +542,13 @@ hipError_t launch_sur_diag(const SoA& in, const SoA& out, const SlotEnt* slots, 
   if (n_blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(sur_diag_kernel, dim3(n_blocks), dim3(kBlock), 0, s, in, out, slots, slot_k, params, mass, work,
                      blocks, diag);
+  return hipGetLastError();
+}
+
+hipError_t launch_sur_health(const SoA& st, const SlotEnt* slots, const BlockRange* blocks, uint32_t n_blocks,
+                             float vmax, float smax, float* out, hipStream_t s) {
+  if (n_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(sur_health_kernel, dim3(n_blocks), dim3(kBlock), 0, s, st, slots, blocks, vmax, smax, out);
   return hipGetLastError();
 }
 

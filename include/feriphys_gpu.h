@@ -215,9 +215,10 @@ void fp_sim_free(fp_sim* sim);
 
 /* ---------------------------------------------------------------------------------------------
  * Learned large-step surrogate.  No counterpart in the reference: the reference only steps with
- * the explicit integrators of state.rs.  A per-vertex network (72 features -> 3 x 128 tanh-GELU ->
- * 3; bf16 weights and activations, f32 accumulation on the matrix cores) stands in for the strut
- * forces over a step of H = ratio * dt:
+ * the explicit integrators of state.rs.  A per-vertex network (72 features -> 3 x 128 GELU -> 3;
+ * bf16 weights and activations, f32 accumulation on the matrix cores) stands in for the strut
+ * forces over a step of H = ratio * dt.  Its GELU is x * clamp(0.5 + x P(x^2), 0, 1) with the
+ * quartic P of surrogate_kernels.hip (within 1.3e-3 of x Phi(x)).  The step:
  *     a  = (gravity + face forces [+ strut forces when base_struts]) / m + out_scale * net(f)
  *     v' = v + H a,   x' = x + H v'
  * then the reference's obstacle collision response over H and pinned records kept.  Features of
@@ -272,10 +273,11 @@ int fp_surrogate_add_samples(fp_surrogate* s, const float* X, const float* T, ui
 int fp_surrogate_train(fp_surrogate* s, uint64_t steps, uint32_t batch, float lr, uint64_t seed, float* losses);
 void fp_surrogate_free(fp_surrogate* s);
 /* Runs n_big * ratio plain steps of the simulation and adds one sample per free vertex and big
- * step b >= 1 ... n_big to the network's training set: the features of the state at big step b
- * with positions and position-difference velocities perturbed by normal noise (`noise` m), and
- * the acceleration that carries it to the plain solver's positions at big step b + 1.  All
- * meshes must use the network's dt. */
+ * step b = 0 ... n_big - 1 to the network's training set: the features of the state at big step b
+ * (positions and position-difference velocities, perturbed by normal noise of `noise` m; the state
+ * before the first step is taken to be at rest) and the acceleration that carries it to the plain
+ * solver's positions at big step b + 1.  A mesh whose plain state stops being finite or reaches
+ * 30 m/s or 100 % strain contributes nothing from then on.  All meshes must use the network's dt. */
 int fp_sim_collect_samples(fp_sim* sim, fp_surrogate* s, uint64_t n_big, float noise, uint64_t seed);
 /* Copies the network into the simulation (NULL detaches).  All meshes must use its dt. */
 int fp_sim_set_surrogate(fp_sim* sim, const fp_surrogate* s);

@@ -6,7 +6,8 @@ library computes:
   features   per vertex and strut-neighbour slot (nbr, rest L): strain_scale * (|d| - L) / L *
              d / |d| and (v_nbr - v) * H / L with d = x_nbr - x (empty slot: zeros)
   forward    z1 = W1 x + b1, h1 = gelu(z1), ... h3 = gelu(z3), y = W4 h3 + b4
-             gelu(z) = 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3)))
+             gelu(z) = z clamp(0.5 + z P(z^2), 0, 1), P the quartic below (the library's
+             polynomial GELU, within 1.3e-3 of z Phi(z))
   loss       mean over rows and the 3 outputs of (y - t)^2
   backward   the analytic gradient of that loss
 With emulate=True the rounding points of the GPU path are applied: weights, inputs, hidden
@@ -14,7 +15,8 @@ activations, dY and dZ rounded to bf16 (nearest even), so the remaining differen
 only f32-versus-f64 accumulation.
 
 Pinned by tests/test_surrogate_oracle.py: the analytic gradient against central finite
-differences (float64, emulate=False), GELU closed forms, hand-computed features.
+differences (float64, emulate=False), the activation against the exact GELU z Phi(z) and its
+saturation, hand-computed features.
 """
 from __future__ import annotations
 
@@ -33,8 +35,9 @@ OFF_W4 = OFF_B3 + HID
 OFF_B4 = OFF_W4 + OUT * HID
 N_PARAMS = OFF_B4 + OUT
 
-_K0 = np.sqrt(2.0 / np.pi)
-_K1 = 0.044715
+# P(u) = sum G[k] u^k, the coefficients the kernels use (f32 values)
+G = np.array([0.39661726355552673, -0.06160679832100868, 0.0071685598231852055, -0.0004622247943188995,
+              1.2258552487764973e-05], dtype=np.float32).astype(np.float64)
 
 
 def bf16(a):
@@ -45,13 +48,22 @@ def bf16(a):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def _poly(u):
+    return (((G[4] * u + G[3]) * u + G[2]) * u + G[1]) * u + G[0]
+
+
 def gelu(z):
-    return 0.5 * z * (1.0 + np.tanh(_K0 * (z + _K1 * z ** 3)))
+    z = np.asarray(z, np.float64)
+    return z * np.clip(0.5 + z * _poly(z * z), 0.0, 1.0)
 
 
 def gelu_grad(z):
-    t = np.tanh(_K0 * (z + _K1 * z ** 3))
-    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * _K0 * (1.0 + 3.0 * _K1 * z * z)
+    z = np.asarray(z, np.float64)
+    u = z * z
+    su = 0.5 + z * _poly(u)
+    dp = ((4 * G[4] * u + 3 * G[3]) * u + 2 * G[2]) * u + G[1]
+    ds = np.where((su > 0) & (su < 1), _poly(u) + 2 * u * dp, 0.0)
+    return np.clip(su, 0.0, 1.0) + z * ds
 
 
 def unpack(p):

@@ -158,7 +158,7 @@ def trained():
     net.collect(sim, n_big=75, noise=1e-3, seed=1)
     n = net.sample_count()
     free = sum(c.rows * c.cols - len(c.pins) for c in sc.cloths)
-    assert n == 75 * free
+    assert 0.9 * 75 * free <= n <= 75 * free  # trajectories that diverge are dropped from then on
     losses = net.train(1500, batch=8192, lr=1e-3, seed=2)
     losses2 = net.train(1000, batch=8192, lr=3e-4, seed=3)
     announce(f"trained on {n} samples: loss {losses[:5].mean():.4f} -> {losses[-50:].mean():.4f} -> "
