@@ -47,7 +47,7 @@ class FpSurrogateConfig(C.Structure):
 
 class FpSurrogateStats(C.Structure):
     _fields_ = [("big_steps", C.c_uint64), ("fallbacks", C.c_uint64), ("mesh_fallbacks", C.c_uint64),
-                ("max_strain_residual", C.c_float), ("max_energy_residual", C.c_float)]
+                ("plain_only", C.c_uint64), ("max_strain_residual", C.c_float), ("max_energy_residual", C.c_float)]
 
 
 FP_SURROGATE_PARAMS = 42755
@@ -117,6 +117,7 @@ SIGNATURES = {
     "fp_surrogate_free": (None, [_P]),
     "fp_sim_collect_samples": (C.c_int, [_P, _P, _U64, C.c_float, _U64]),
     "fp_sim_set_surrogate": (C.c_int, [_P, _P]),
+    "fp_sim_set_surrogate_shape": (C.c_int, [_P, C.c_int32]),
     "fp_sim_step_surrogate": (C.c_int, [_P, _U64, C.POINTER(FpSurrogateStats)]),
     "fp_sim_time_surrogate": (C.c_int, [_P, _U64, _FP, _FP, C.POINTER(FpSurrogateStats)]),
     "fp_sim_surrogate_features": (C.c_int, [_P, _U64, _FP]),

@@ -148,6 +148,10 @@ struct fp_sim {
     DBuf<uint32_t> mesh_of, ids;
     std::vector<fp::SlotEnt> h_slots;
     int grid = 256;
+    int variant = 2;  // inference kernel: vertices per wave / 16
+    // per mesh: big steps left to run plain before the network is tried again, and the current
+    // backoff length (doubles on each rejection up to 16, cleared by an accepted step)
+    std::vector<uint32_t> skip, backoff;
   } sur;
 
   void drop_graphs() {
@@ -1194,12 +1198,25 @@ fp::FeatParams feat_params(const fp_surrogate_config& c) {
 int sur_big_step(fp_sim* s, fp_surrogate_stats& st, float* mlp_ms) {
   auto& S = s->sur;
   const uint32_t nblk = (uint32_t)s->h_blocks.size();
+  const size_t M = s->meshes.size();
+  S.skip.resize(M, 0);
+  S.backoff.resize(M, 0);
+  bool all_skip = true;
+  for (size_t m = 0; m < M; ++m) all_skip = all_skip && S.skip[m] > 0;
+  if (all_skip) {  // every mesh is backing off: plain steps, no network attempt
+    for (size_t m = 0; m < M; ++m) --S.skip[m];
+    ++st.big_steps;
+    ++st.fallbacks;
+    ++st.plain_only;
+    st.mesh_fallbacks += M;
+    return run_steps(s, (uint64_t)S.cfg.ratio);
+  }
   fp::StepArgs a = s->args(false);
   a.params = S.big_params.p;
   HIP_TRY(fp::launch_faces(a, s->csr(), s->all_face_blocks.p, (uint32_t)s->h_face_blocks.size(), false, s->stream));
   if (mlp_ms) HIP_TRY(hipEventRecord(s->ev0, s->stream));
   HIP_TRY(fp::launch_sur_mlp(a.in, S.slots.p, (uint32_t)s->N, S.image.p, feat_params(S.cfg), S.cfg.out_scale, S.da.p,
-                             S.grid, s->stream));
+                             S.grid, S.variant, s->stream));
   if (mlp_ms) HIP_TRY(hipEventRecord(s->ev1, s->stream));
   HIP_TRY(fp::launch_sur_integrate(a, s->csr(), s->all_blocks.p, nblk, S.da.p, S.work.p, S.cfg.base_struts != 0,
                                    s->stream));
@@ -1213,7 +1230,6 @@ int sur_big_step(fp_sim* s, fp_surrogate_stats& st, float* mlp_ms) {
     HIP_TRY(hipEventElapsedTime(&t, s->ev0, s->ev1));
     *mlp_ms += t;
   }
-  const size_t M = s->meshes.size();
   std::vector<double> eo(M, 0.0), en(M, 0.0), em(M, 0.0);
   std::vector<float> ds(M, 0.0f);
   for (uint32_t b = 0; b < nblk; ++b) {
@@ -1227,9 +1243,17 @@ int sur_big_step(fp_sim* s, fp_surrogate_stats& st, float* mlp_ms) {
   std::vector<uint8_t> bad(M, 0);
   size_t nbad = 0;
   for (size_t m = 0; m < M; ++m) {
+    if (S.skip[m] > 0) {  // backing off: plain, whatever its residuals
+      --S.skip[m];
+      bad[m] = 1;
+      ++nbad;
+      continue;
+    }
     const float er = (float)((en[m] - eo[m]) / std::max(em[m], 1e-30));
     bad[m] = !(ds[m] <= S.cfg.strain_limit) || !(er <= S.cfg.energy_limit);
     nbad += bad[m];
+    S.backoff[m] = bad[m] ? std::min<uint32_t>(S.backoff[m] ? 2 * S.backoff[m] : 1, 16) : 0;
+    S.skip[m] = bad[m] ? S.backoff[m] : 0;
     st.max_strain_residual = (ds[m] > st.max_strain_residual || ds[m] != ds[m]) ? ds[m] : st.max_strain_residual;
     st.max_energy_residual = (er > st.max_energy_residual || er != er) ? er : st.max_energy_residual;
   }
@@ -1291,7 +1315,16 @@ int fp_sim_set_surrogate(fp_sim* s, const fp_surrogate* n) {
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   s->sur.grid = cus > 0 ? cus : 256;
   s->sur.cfg = n->cfg;
+  s->sur.skip.assign(s->meshes.size(), 0);
+  s->sur.backoff.assign(s->meshes.size(), 0);
   s->sur.on = true;
+  return FP_OK;
+}
+
+int fp_sim_set_surrogate_shape(fp_sim* s, int32_t vertices_per_wave) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (vertices_per_wave != 32 && vertices_per_wave != 64) return fail(FP_EINVAL, "vertices_per_wave must be 32 or 64");
+  s->sur.variant = vertices_per_wave / 16;
   return FP_OK;
 }
 
@@ -1425,7 +1458,7 @@ int fp_sim_surrogate_accel(fp_sim* s, uint64_t mesh, float* accel) {
   auto& S = s->sur;
   fp::StepArgs a = s->args(false);
   HIP_TRY(fp::launch_sur_mlp(a.in, S.slots.p, (uint32_t)s->N, S.image.p, feat_params(S.cfg), S.cfg.out_scale, S.da.p,
-                             S.grid, s->stream));
+                             S.grid, S.variant, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   const MeshSlot& m = s->meshes[mesh];
   HIP_TRY(hipMemcpy(accel, S.da.p + 3 * m.vbase, 3 * m.host.n * sizeof(float), hipMemcpyDeviceToHost));

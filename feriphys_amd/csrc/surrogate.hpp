@@ -58,8 +58,9 @@ struct DiagOut {
   float max_dstrain, e_old, e_new, emech_old;
 };
 
+// variant: vertices per wave / 16 (2: 1024-thread workgroups, the default; 4: 512 threads)
 hipError_t launch_sur_mlp(const SoA& st, const SlotEnt* slots, uint32_t n, const void* image, FeatParams fp,
-                          float out_scale, float* da, int grid, hipStream_t s);
+                          float out_scale, float* da, int grid, int variant, hipStream_t s);
 // Big-step update of the vertices in `blocks` from `a.in` to `a.out` with acceleration
 // F_ext / m + da (F_ext: gravity and face forces, plus strut forces when `struts`); `work` gets
 // each vertex's face-force work over the step.

@@ -45,20 +45,24 @@ __device__ __forceinline__ void store_state(const SoA& o, uint32_t i, f3 x, f3 v
   o.vx[i] = v.x; o.vy[i] = v.y; o.vz[i] = v.z;
 }
 
-// The 6 features of one slot: strain_scale * strain * d/|d| and (v_nbr - v) * H / L.
-__device__ __forceinline__ void slot_feat(const SoA& S, f3 p, f3 v, SlotEnt e, FeatParams fp, float* f) {
-  if (e.nbr == kNoSlot) {
-    for (int k = 0; k < 6; ++k) f[k] = 0.0f;
-    return;
-  }
-  const f3 d = sub(ld3(S.x, S.y, S.z, e.nbr), p);
-  const f3 w = ld3(S.vx, S.vy, S.vz, e.nbr);
+// The 6 features of one slot: strain_scale * strain * d/|d| and (v_nbr - v) * H / L, from the
+// neighbour's position q and velocity w; zeros when `has` is false.  Branch-free, so that a caller
+// can issue every load of a pass before the first feature is computed.
+__device__ __forceinline__ void slot_feat_from(f3 p, f3 v, f3 q, f3 w, float rest, bool has, FeatParams fp, float* f) {
+  const f3 d = sub(q, p);
   const float len = __builtin_sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
-  const float il = 1.0f / e.rest;
-  const float st = len > 0.0f ? fp.strain_scale * ((len - e.rest) * il) / len : 0.0f;
+  const float il = __builtin_amdgcn_rcpf(rest);
+  const float st = (has && len > 0.0f) ? fp.strain_scale * ((len - rest) * il) * __builtin_amdgcn_rcpf(len) : 0.0f;
+  const float hv = has ? fp.H * il : 0.0f;
   f[0] = st * d.x; f[1] = st * d.y; f[2] = st * d.z;
-  const float hv = fp.H * il;
   f[3] = (w.x - v.x) * hv; f[4] = (w.y - v.y) * hv; f[5] = (w.z - v.z) * hv;
+}
+
+__device__ __forceinline__ void slot_feat(const SoA& S, f3 p, f3 v, uint32_t self, SlotEnt e, FeatParams fp,
+                                          float* f) {
+  const bool has = e.nbr != kNoSlot;
+  const uint32_t nb = has ? e.nbr : self;
+  slot_feat_from(p, v, ld3(S.x, S.y, S.z, nb), ld3(S.vx, S.vy, S.vz, nb), has ? e.rest : 1.0f, has, fp, f);
 }
 
 __device__ __forceinline__ uint64_t mix64(uint64_t z) {
@@ -70,24 +74,24 @@ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
 __device__ __forceinline__ float unit_open(uint64_t h) { return ((float)(h >> 40) + 0.5f) * (1.0f / 16777216.0f); }
 
 // ---------------------------------------------------------------------------------------------
-// Inference: 512 threads = 8 waves, each wave 64 vertices (4 MFMA column tiles of 16) per pass.
+// Inference: a workgroup takes 512 vertices per pass, NT MFMA column tiles of 16 vertices per wave
+// (NT = 2: 16 waves of 32 vertices, 4 waves per SIMD within 128 registers; NT = 4: 8 waves of 64).
 // Neurons are the MFMA rows, vertices the columns, so a layer's f32 output tile already holds, per
 // lane, 4 consecutive neurons of one vertex; two such tiles form the next layer's 8-element bf16
 // operand once the weights' K index is permuted the same way (done when the image is packed).  The
 // activations never leave registers; the weights stay in LDS for the whole (persistent) launch.
 
-constexpr int kMlpThreads = 512;
 constexpr int kMlpTile = 512;
 
 // acc = W B + bias (the bias is the accumulators' initial value)
-template <int KB, int LDW>
+template <int KB, int LDW, int NT>
 __device__ __forceinline__ void mfma_layer(const __bf16* __restrict__ W, const float* __restrict__ bias,
-                                           const bf16x8 (&B)[KB][4], f32x4 (&acc)[8][4], int col, int g) {
+                                           const bf16x8 (&B)[KB][NT], f32x4 (&acc)[8][NT], int col, int g) {
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
     const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = b;
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = b;
   }
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb)
@@ -95,24 +99,27 @@ __device__ __forceinline__ void mfma_layer(const __bf16* __restrict__ W, const f
     for (int mt = 0; mt < 8; ++mt) {
       const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * mt + col) * LDW + kb * 32 + g * 8);
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = mfma(a, B[kb][nt], acc[mt][nt]);
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(a, B[kb][nt], acc[mt][nt]);
     }
 }
 
-__device__ __forceinline__ void gelu_pack(const f32x4 (&acc)[8][4], bf16x8 (&out)[4][4]) {
+template <int NT>
+__device__ __forceinline__ void gelu_pack(const f32x4 (&acc)[8][NT], bf16x8 (&out)[4][NT]) {
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt)
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt)
+    for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
       for (int r = 0; r < 4; ++r) out[mt >> 1][nt][(mt & 1) * 4 + r] = (__bf16)gelu(acc[mt][nt][r]);
 }
 
-__global__ void __launch_bounds__(kMlpThreads) sur_mlp_kernel(SoA S, const SlotEnt* __restrict__ slots, uint32_t n,
-                                                              const uint4* __restrict__ image, FeatParams fp,
-                                                              float out_scale, float* __restrict__ da) {
+template <int NT>
+__global__ void __launch_bounds__(2048 / NT) sur_mlp_kernel(SoA S, const SlotEnt* __restrict__ slots, uint32_t n,
+                                                            const uint4* __restrict__ image, FeatParams fp,
+                                                            float out_scale, float* __restrict__ da) {
+  constexpr int kThreads = 2048 / NT;
   __shared__ uint4 s_img[kImgBytes / 16];
-  for (int i = threadIdx.x; i < kImgBytes / 16; i += kMlpThreads) s_img[i] = image[i];
+  for (int i = threadIdx.x; i < kImgBytes / 16; i += kThreads) s_img[i] = image[i];
   __syncthreads();
   const char* img = reinterpret_cast<const char*>(s_img);
   const __bf16* W1 = reinterpret_cast<const __bf16*>(img + kImgW1);
@@ -122,49 +129,62 @@ __global__ void __launch_bounds__(kMlpThreads) sur_mlp_kernel(SoA S, const SlotE
   const float* Bias = reinterpret_cast<const float*>(img + kImgB);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, col = lane & 15, g = lane >> 4;
   for (uint32_t t0 = blockIdx.x * kMlpTile; t0 < n; t0 += gridDim.x * kMlpTile) {
-    const uint32_t v0 = t0 + wave * 64;
+    const uint32_t v0 = t0 + wave * 16 * NT;
     if (v0 >= n) continue;
     // features: lane group g computes slots 3g..3g+2 of its vertex (18 values, then 6 zeros), which
-    // are its 8-element K slices of the three 32-wide K blocks of layer 1
-    bf16x8 B1[3][4];
+    // are its 8-element K slices of the three 32-wide K blocks of layer 1.  Every load is
+    // unconditional (past-the-end columns read vertex n-1, empty slots the vertex itself; both are
+    // zeroed), so a pass's loads can all be in flight together.
+    bf16x8 B1[3][NT];
+    SlotEnt se[NT][3];
+    f3 p[NT], v[NT];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      const uint32_t i = v0 + nt * 16 + col;
+    for (int nt = 0; nt < NT; ++nt) {
+      const uint32_t iv = v0 + nt * 16 + col;
+      const uint32_t i = iv < n ? iv : n - 1;
+      p[nt] = ld3(S.x, S.y, S.z, i);
+      v[nt] = ld3(S.vx, S.vy, S.vz, i);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        se[nt][j] = slots[(size_t)i * kSlots + 3 * g + j];
+        if (iv >= n) se[nt][j].nbr = kNoSlot;
+      }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const uint32_t iv = v0 + nt * 16 + col;
+      const uint32_t i = iv < n ? iv : n - 1;
       float f[24];
 #pragma unroll
-      for (int k = 0; k < 24; ++k) f[k] = 0.0f;
-      if (i < n) {
-        const f3 p = ld3(S.x, S.y, S.z, i), v = ld3(S.vx, S.vy, S.vz, i);
-        const SlotEnt* se = slots + (size_t)i * kSlots + 3 * g;
+      for (int j = 0; j < 3; ++j) slot_feat(S, p[nt], v[nt], i, se[nt][j], fp, f + 6 * j);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) slot_feat(S, p, v, se[j], fp, f + 6 * j);
-      }
+      for (int k = 18; k < 24; ++k) f[k] = 0.0f;
 #pragma unroll
       for (int kb = 0; kb < 3; ++kb)
 #pragma unroll
         for (int s = 0; s < 8; ++s) B1[kb][nt][s] = (__bf16)f[kb * 8 + s];
     }
-    f32x4 acc[8][4];
-    bf16x8 Bh[4][4];
-    mfma_layer<3, kLd1>(W1, Bias, B1, acc, col, g);
-    gelu_pack(acc, Bh);
-    mfma_layer<4, kLd>(W2, Bias + kHid, Bh, acc, col, g);
-    gelu_pack(acc, Bh);
-    mfma_layer<4, kLd>(W3, Bias + 2 * kHid, Bh, acc, col, g);
-    gelu_pack(acc, Bh);
-    f32x4 o[4];
+    f32x4 acc[8][NT];
+    bf16x8 Bh[4][NT];
+    mfma_layer<3, kLd1, NT>(W1, Bias, B1, acc, col, g);
+    gelu_pack<NT>(acc, Bh);
+    mfma_layer<4, kLd, NT>(W2, Bias + kHid, Bh, acc, col, g);
+    gelu_pack<NT>(acc, Bh);
+    mfma_layer<4, kLd, NT>(W3, Bias + 2 * kHid, Bh, acc, col, g);
+    gelu_pack<NT>(acc, Bh);
+    f32x4 o[NT];
 #pragma unroll
-    for (int nt = 0; nt < 4; ++nt) o[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    for (int nt = 0; nt < NT; ++nt) o[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int kb = 0; kb < 4; ++kb) {
       const bf16x8 a = *reinterpret_cast<const bf16x8*>(W4 + col * kLd + kb * 32 + g * 8);
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) o[nt] = mfma(a, Bh[kb][nt], o[nt]);
+      for (int nt = 0; nt < NT; ++nt) o[nt] = mfma(a, Bh[kb][nt], o[nt]);
     }
     if (g == 0) {
       const float b0 = Bias[3 * kHid], b1 = Bias[3 * kHid + 1], b2 = Bias[3 * kHid + 2];
 #pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
+      for (int nt = 0; nt < NT; ++nt) {
         const uint32_t i = v0 + nt * 16 + col;
         if (i < n) {
           da[3 * (size_t)i] = out_scale * (o[nt][0] + b0);
@@ -233,29 +253,49 @@ __global__ void __launch_bounds__(kBlock) sur_diag_kernel(SoA I, SoA O, const Sl
                                                            const BlockRange* __restrict__ blocks, DiagOut* __restrict__ out) {
   __shared__ float s_ds[kBlock], s_eo[kBlock], s_en[kBlock], s_em[kBlock];
   const BlockRange br = blocks[blockIdx.x];
-  const uint32_t i = br.begin + threadIdx.x;
+  const uint32_t iv = br.begin + threadIdx.x;
+  const bool valid = iv < br.end;
+  const uint32_t i = valid ? iv : br.begin;
   float ds = 0.0f, eo = 0.0f, en = 0.0f, em = 0.0f;
-  if (i < br.end) {
+  {
     const StepParams P = params[br.mesh];
     const float m = P.uniform_mass ? P.mass : mass[i];
     const f3 po = ld3(I.x, I.y, I.z, i), pn = ld3(O.x, O.y, O.z, i);
     const f3 vo = ld3(I.vx, I.vy, I.vz, i), vn = ld3(O.vx, O.vy, O.vz, i);
-    float so = 0.0f, sn = 0.0f;
+    // all slot entries first, then all neighbour positions (empty slots read the vertex itself), so
+    // the loads are in flight together
+    SlotEnt e[kSlots];
+    float k[kSlots];
+#pragma unroll
     for (int j = 0; j < kSlots; ++j) {
-      const SlotEnt e = slots[(size_t)i * kSlots + j];
-      if (e.nbr == kNoSlot) continue;
-      const float k = slot_k[(size_t)i * kSlots + j];
-      const f3 d_o = sub(ld3(I.x, I.y, I.z, e.nbr), po), d_n = sub(ld3(O.x, O.y, O.z, e.nbr), pn);
-      const float lo = __builtin_sqrtf(dot(d_o, d_o)) - e.rest, ln = __builtin_sqrtf(dot(d_n, d_n)) - e.rest;
-      const float d = __builtin_fabsf(ln - lo) / e.rest;
+      e[j] = slots[(size_t)i * kSlots + j];
+      k[j] = slot_k[(size_t)i * kSlots + j];
+    }
+    f3 qo[kSlots], qn[kSlots];
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const uint32_t nb = e[j].nbr != kNoSlot ? e[j].nbr : i;
+      qo[j] = ld3(I.x, I.y, I.z, nb);
+      qn[j] = ld3(O.x, O.y, O.z, nb);
+    }
+    float so = 0.0f, sn = 0.0f;
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j) {
+      const bool has = e[j].nbr != kNoSlot;
+      const f3 d_o = sub(qo[j], po), d_n = sub(qn[j], pn);
+      const float r = has ? e[j].rest : 1.0f;
+      const float lo = __builtin_sqrtf(dot(d_o, d_o)) - r, ln = __builtin_sqrtf(dot(d_n, d_n)) - r;
+      const float d = has ? __builtin_fabsf(ln - lo) * __builtin_amdgcn_rcpf(r) : 0.0f;
       ds = (d > ds || d != d) ? d : ds;
-      so += 0.25f * k * lo * lo;
-      sn += 0.25f * k * ln * ln;
+      const float kk = has ? 0.25f * k[j] : 0.0f;
+      so += kk * lo * lo;
+      sn += kk * ln * ln;
     }
     const float ko = 0.5f * m * dot(vo, vo), kn = 0.5f * m * dot(vn, vn);
     em = ko + so + kn + sn;
     eo = ko + so - dot(P.gravity, po);
     en = kn + sn - dot(P.gravity, pn) - work[i];
+    if (!valid) ds = eo = en = em = 0.0f;
   }
   s_ds[threadIdx.x] = ds; s_eo[threadIdx.x] = eo; s_en[threadIdx.x] = en; s_em[threadIdx.x] = em;
   __syncthreads();
@@ -308,7 +348,7 @@ __global__ void __launch_bounds__(kBlock) sur_features_kernel(SoA S, const SlotE
   if (j >= n) return;
   const uint32_t i = ids[j];
   const f3 p = ld3(S.x, S.y, S.z, i), v = ld3(S.vx, S.vy, S.vz, i);
-  for (int s = 0; s < kSlots; ++s) slot_feat(S, p, v, slots[(size_t)i * kSlots + s], fp, out + (size_t)j * kFeat + 6 * s);
+  for (int s = 0; s < kSlots; ++s) slot_feat(S, p, v, i, slots[(size_t)i * kSlots + s], fp, out + (size_t)j * kFeat + 6 * s);
 }
 
 __global__ void __launch_bounds__(kBlock) sur_noise_kernel(Pos3 prev, Pos3 cur, SoA o, const uint32_t* __restrict__ pins,
@@ -345,7 +385,7 @@ __global__ void __launch_bounds__(kBlock) sur_samples_kernel(StepArgs A, CsrView
   const f3 p = ld3(S.x, S.y, S.z, i), v = ld3(S.vx, S.vy, S.vz, i);
   float f[kFeat];
   bool ok = true;
-  for (int s = 0; s < kSlots; ++s) slot_feat(S, p, v, slots[(size_t)i * kSlots + s], fp, f + 6 * s);
+  for (int s = 0; s < kSlots; ++s) slot_feat(S, p, v, i, slots[(size_t)i * kSlots + s], fp, f + 6 * s);
   for (int c = 0; c < kFeat; ++c) ok = ok && __builtin_isfinite(f[c]);
   const float m = P.uniform_mass ? P.mass : A.mass[i];
   f3 F = P.gravity;
@@ -518,11 +558,16 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock
 }  // namespace
 
 hipError_t launch_sur_mlp(const SoA& st, const SlotEnt* slots, uint32_t n, const void* image, FeatParams fp,
-                          float out_scale, float* da, int grid, hipStream_t s) {
+                          float out_scale, float* da, int grid, int variant, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const int tiles = (int)((n + kMlpTile - 1) / kMlpTile);
-  hipLaunchKernelGGL(sur_mlp_kernel, dim3(grid < tiles ? grid : tiles), dim3(kMlpThreads), 0, s, st, slots, n,
-                     reinterpret_cast<const uint4*>(image), fp, out_scale, da);
+  const dim3 g(grid < tiles ? grid : tiles);
+  if (variant == 4)
+    hipLaunchKernelGGL(sur_mlp_kernel<4>, g, dim3(512), 0, s, st, slots, n, reinterpret_cast<const uint4*>(image), fp,
+                       out_scale, da);
+  else
+    hipLaunchKernelGGL(sur_mlp_kernel<2>, g, dim3(1024), 0, s, st, slots, n, reinterpret_cast<const uint4*>(image), fp,
+                       out_scale, da);
   return hipGetLastError();
 }
 

@@ -335,6 +335,9 @@ class Simulation(_Handle):
         """Copies the network into the simulation (None detaches)."""
         check(lib().fp_sim_set_surrogate(self.handle, None if net is None else net.handle))
 
+    def set_surrogate_shape(self, vertices_per_wave: int):
+        check(lib().fp_sim_set_surrogate_shape(self.handle, vertices_per_wave))
+
     def step_surrogate(self, n_big: int = 1):
         from .surrogate import SurrogateStats
 

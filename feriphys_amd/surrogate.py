@@ -50,12 +50,13 @@ class SurrogateStats:
     big_steps: int
     fallbacks: int
     mesh_fallbacks: int
+    plain_only: int
     max_strain_residual: float
     max_energy_residual: float
 
     @staticmethod
     def from_c(s: _lib.FpSurrogateStats) -> "SurrogateStats":
-        return SurrogateStats(s.big_steps, s.fallbacks, s.mesh_fallbacks, s.max_strain_residual,
+        return SurrogateStats(s.big_steps, s.fallbacks, s.mesh_fallbacks, s.plain_only, s.max_strain_residual,
                               s.max_energy_residual)
 
 

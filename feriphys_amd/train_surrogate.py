@@ -3,8 +3,9 @@
     python -m feriphys_amd.train_surrogate --out feriphys_amd/weights/cloth_surrogate.npy
 
 Data: the plain solver's trajectories (600 steps at dt = 1/600 s) of seeded training cloths
-(workloads.surrogate_training_scene: one pinned edge, random wind), sampled every 8 steps, with
-normal position noise on the inputs.  Training: Adam on the library's bf16 MFMA kernels, in three
+(workloads.surrogate_training_scene: one pinned edge, random wind; 2,000 with sides in [12, 96]
+and a few with sides in [192, 512]), sampled every 8 steps, with normal position noise on the
+inputs.  Training: Adam on the library's bf16 MFMA kernels, in three
 learning-rate phases.  Evaluation: the 32 x 32 edge-pinned cloth stepped 75 big steps with the
 fallback off, against 600 plain steps.
 """
@@ -51,6 +52,8 @@ def main():
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--groups", type=int, default=4, help="trajectories are simulated in this many batches")
+    ap.add_argument("--large", type=int, default=16,
+                    help="extra cloths with sides in [192, 512] (the tension of long hanging cloths)")
     ap.add_argument("--out", default="feriphys_amd/weights/cloth_surrogate.npy")
     args = ap.parse_args()
 
@@ -64,6 +67,13 @@ def main():
         net.collect(sim, n_big=args.big, noise=args.noise, seed=args.seed * 7919 + g)
         del sim
         print(f"group {g}: {n} cloths, {net.sample_count()} samples so far ({time.time() - t0:.1f} s)", flush=True)
+    if args.large:
+        sc = wl.surrogate_training_scene(n_cloths=args.large, seed=7777 + args.seed, smin=192, smax=512)
+        sim = wl.build_gpu(sc)
+        net.collect(sim, n_big=args.big, noise=args.noise, seed=args.seed * 7919 + 99)
+        del sim
+        print(f"large cloths: {args.large}, {net.sample_count()} samples so far ({time.time() - t0:.1f} s)",
+              flush=True)
     t1 = time.time()
     phases = [(0.45, 1e-3), (0.35, 3e-4), (0.2, 1e-4)]
     log = []

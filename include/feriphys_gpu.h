@@ -228,7 +228,10 @@ void fp_sim_free(fp_sim* sim);
  * the surrogate state are position differences over H.  A big step is redone with `ratio` plain
  * steps for every mesh where a strut strain changed by more than strain_limit, or the energy
  * (kinetic + spring + gravity, net of the face forces' work) rose by more than energy_limit times
- * the kinetic + spring energy before and after.
+ * the kinetic + spring energy before and after.  A mesh whose step was redone runs plain for the
+ * next 1, 2, 4, ... 16 big steps (doubling on each further rejection, reset by an accepted step)
+ * before the network is tried on it again; when every mesh is backing off no network step is
+ * attempted.
  * ------------------------------------------------------------------------------------------- */
 typedef struct fp_surrogate fp_surrogate;
 typedef struct fp_surrogate_config {
@@ -244,6 +247,7 @@ typedef struct fp_surrogate_stats {
   uint64_t big_steps;      /* big steps taken */
   uint64_t fallbacks;      /* big steps in which at least one mesh fell back to plain steps */
   uint64_t mesh_fallbacks; /* (mesh, big step) pairs that fell back */
+  uint64_t plain_only;     /* big steps run plain without a network attempt (all meshes backing off) */
   float max_strain_residual;
   float max_energy_residual;
 } fp_surrogate_stats;
@@ -281,6 +285,9 @@ void fp_surrogate_free(fp_surrogate* s);
 int fp_sim_collect_samples(fp_sim* sim, fp_surrogate* s, uint64_t n_big, float noise, uint64_t seed);
 /* Copies the network into the simulation (NULL detaches).  All meshes must use its dt. */
 int fp_sim_set_surrogate(fp_sim* sim, const fp_surrogate* s);
+/* Shape of the network's inference kernel: 32 vertices per wave (1024-thread workgroups, four
+ * waves per SIMD; the default) or 64 (512-thread workgroups, two waves per SIMD). */
+int fp_sim_set_surrogate_shape(fp_sim* sim, int32_t vertices_per_wave);
 /* n_big surrogate steps (each ratio * dt), with per-mesh fallback to plain steps.  Blocks. */
 int fp_sim_step_surrogate(fp_sim* sim, uint64_t n_big, fp_surrogate_stats* stats);
 /* The same, timed with HIP events on the simulation's stream: total_ms for the whole run,

@@ -119,9 +119,10 @@ def test_fallback_everywhere_is_the_plain_solver():
     sc = wl.config0(steps=1, seed=7, rows=12, cols=10, pins="edge")
     a, b = wl.build_gpu(sc), wl.build_gpu(sc)
     a.set_surrogate(fa.Surrogate(fa.SurrogateConfig(strain_limit=-1.0), seed=1))
-    st = a.step_surrogate(3)
-    assert st.big_steps == 3 and st.fallbacks == 3 and st.mesh_fallbacks == 3
-    b.step(24)
+    st = a.step_surrogate(5)
+    # attempt, back off 1, attempt, back off 2 (2 steps): 5 big steps, 3 without a network attempt
+    assert st.big_steps == 5 and st.fallbacks == 5 and st.mesh_fallbacks == 5 and st.plain_only == 3
+    b.step(40)
     assert (bits(a.get_points(0)) == bits(b.get_points(0))).all()
 
 
