@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VERTEX_STEP = 48
+# HBM bytes per launch of the default step kernel (grid_tiled_kernel<ExactMath, 16, 16>) on the
+# 1024 x 1024 cloth, from rocprofv3 FETCH_SIZE (13,230 KiB x 2.000) and WRITE_SIZE (24,580 KiB x
+# 1.000), each scaled by a 256 MiB device copy's reading (profiles/grid_tiled_exact_cloth1024.md)
+MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13230 * 2.0 + 24580 * 1.0) * 1024}
 
 
 def main():
@@ -43,6 +47,8 @@ def main():
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph (batched)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "feriphys_amd", "weights", "cloth_surrogate.npy"),
                     help="surrogate network for the cloth workload's surrogate leg ('' to skip it)")
+    ap.add_argument("--sur-shape", type=int, choices=[32, 64], default=32,
+                    help="vertices per wave of the surrogate network kernel")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -136,7 +142,8 @@ def main():
         "vertex_steps_per_s": args.steps * N / wall,
         "surrogate": sur,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.n, args.math, args.kernel, args.tile)),
                      "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": BYTES_PER_VERTEX_STEP * N},
         "cpu_baseline": cpu,
     }
@@ -161,6 +168,7 @@ def surrogate_leg(args, np, fa, wl, kern, ref, initial, N, dt, plain_sim_s_per_s
     sim = wl.build_gpu(wl.config1(n=args.n, steps=args.steps))
     sim.set_kernel(kern, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
     sim.set_surrogate(net)
+    sim.set_surrogate_shape(args.sur_shape)
     if args.warmup:
         sim.step_surrogate(max(1, args.warmup // ratio))
         sim.set_points(0, initial)
