@@ -116,6 +116,8 @@ SIGNATURES = {
     "fp_surrogate_train": (C.c_int, [_P, _U64, C.c_uint32, C.c_float, _U64, _FP]),
     "fp_surrogate_free": (None, [_P]),
     "fp_sim_collect_samples": (C.c_int, [_P, _P, _U64, C.c_float, _U64]),
+    "fp_sim_apply_torsional_forces": (C.c_int, [_P, _U64]),
+    "fp_sim_set_torsion": (C.c_int, [_P, C.c_int32]),
     "fp_sim_set_surrogate": (C.c_int, [_P, _P]),
     "fp_sim_set_surrogate_shape": (C.c_int, [_P, C.c_int32]),
     "fp_sim_step_surrogate": (C.c_int, [_P, _U64, C.POINTER(FpSurrogateStats)]),

@@ -48,6 +48,20 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
     const StrutEnt se = T.s_ent[e];
     strut_accumulate<M>(F, p, v, ld3(S.x, S.y, S.z, se.nbr), ld3(S.vx, S.vy, S.vz, se.nbr), se.k, se.c, se.len);
   }
+  // apply_torsional_forces (:553-627), when the step includes it (the reference comments the call
+  // out at :518-519): the vertex's hinge forces summed in strut order from zero, then added once
+  if (T.t_row != nullptr) {
+    uint32_t e = T.t_row[i];
+    const uint32_t ee = T.t_row[i + 1];
+    if (e < ee) {
+      f3 ts = f3{0.0f, 0.0f, 0.0f};
+      for (; e < ee; ++e) {
+        const float4 hf = T.hinge_f[T.t_ent[e]];
+        ts = add(ts, f3{hf.x, hf.y, hf.z});
+      }
+      F = add(F, ts);
+    }
+  }
   // apply_face_forces (:629-654): the face kernel has computed each face's drag + lift and corner
   // weights; this vertex adds weight * force for its faces in face order
   for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {
@@ -82,6 +96,41 @@ __global__ void __launch_bounds__(kBlock) face_kernel(StepArgs A, CsrView T, con
   T.face_w[3 * f] = corner_weight<M>(p0, p1, p2);
   T.face_w[3 * f + 1] = corner_weight<M>(p1, p2, p0);
   T.face_w[3 * f + 2] = corner_weight<M>(p2, p0, p1);
+}
+
+// The four forces of each torsional hinge, once per hinge.
+template <class M>
+__global__ void __launch_bounds__(kBlock) hinge_kernel(StepArgs A, CsrView T, const HingeEnt* __restrict__ hinges,
+                                                        uint32_t first, uint32_t count) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t hgi = first + k;
+  const HingeEnt H = hinges[hgi];
+  const SoA& S = A.in;
+  f3 out[4];
+  hinge_forces<M>(ld3(S.x, S.y, S.z, H.v[0]), ld3(S.x, S.y, S.z, H.v[1]), ld3(S.x, S.y, S.z, H.v[2]),
+                  ld3(S.x, S.y, S.z, H.v[3]), ld3(S.vx, S.vy, S.vz, H.v[2]), ld3(S.vx, S.vy, S.vz, H.v[3]), H.k, H.c,
+                  H.rest, out);
+  for (int r = 0; r < 4; ++r) T.hinge_f[4 * hgi + r] = make_float4(out[r].x, out[r].y, out[r].z, 0.0f);
+}
+
+// Split API: F += the vertex's hinge forces (vertices with no hinge are left untouched).
+__global__ void __launch_bounds__(kBlock) torsion_gather_kernel(StepArgs A, CsrView T,
+                                                                 const BlockRange* __restrict__ blocks) {
+  const BlockRange br = blocks[blockIdx.x];
+  const uint32_t i = br.begin + threadIdx.x;
+  if (i >= br.end) return;
+  uint32_t e = T.t_row[i];
+  const uint32_t ee = T.t_row[i + 1];
+  if (e == ee) return;
+  f3 ts = f3{0.0f, 0.0f, 0.0f};
+  for (; e < ee; ++e) {
+    const float4 hf = T.hinge_f[T.t_ent[e]];
+    ts = add(ts, f3{hf.x, hf.y, hf.z});
+  }
+  A.fx[i] = A.fx[i] + ts.x;
+  A.fy[i] = A.fy[i] + ts.y;
+  A.fz[i] = A.fz[i] + ts.z;
 }
 
 // Grid cloth built by Cloth::new (springy/cloth.rs:20-113).  Vertex (r, c) of an R x C grid.
@@ -241,6 +290,24 @@ hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s) {
   uint64_t blocks = (n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   fill_kernel<<<dim3((uint32_t)blocks), dim3(256), 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_hinges(const StepArgs& a, const CsrView& t, const HingeEnt* hinges, uint32_t first, uint32_t count,
+                         bool exact, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  const dim3 g((count + kBlock - 1) / kBlock);
+  if (exact)
+    hipLaunchKernelGGL(hinge_kernel<ExactMath>, g, dim3(kBlock), 0, s, a, t, hinges, first, count);
+  else
+    hipLaunchKernelGGL(hinge_kernel<FastMath>, g, dim3(kBlock), 0, s, a, t, hinges, first, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_torsion_gather(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks,
+                                 hipStream_t s) {
+  if (n_blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(torsion_gather_kernel, dim3(n_blocks), dim3(kBlock), 0, s, a, t, blocks);
   return hipGetLastError();
 }
 

@@ -20,6 +20,13 @@ struct StrutEnt {
   float len, k, c;
 };
 
+// One torsional hinge: the strut (x0, x1), the vertex x2 of its first face and x3 of its second
+// that are not on the strut (utils::tuple_difference), its spring constant, damping and rest angle.
+struct HingeEnt {
+  uint32_t v[4];
+  float k, c, rest, pad;
+};
+
 // Vertex-centric CSR of a packed set of meshes (global vertex and face ids).
 struct CsrView {
   const uint32_t* s_row;  // N+1: incident struts of each vertex, in strut-index order
@@ -29,6 +36,11 @@ struct CsrView {
   const uint32_t* f_v;    // 3 global vertex ids per face
   float4* face_sum;       // per face: drag + lift (x, y, z), written by the face kernel
   float* face_w;          // per face corner: angle / PI, written by the face kernel
+  // torsional hinges in the step (nullptr: none): per vertex its (hinge << 2 | role) entries in
+  // strut order, and the 4 forces per hinge written by the hinge kernel
+  const uint32_t* t_row;
+  const uint32_t* t_ent;
+  float4* hinge_f;
 };
 
 // A Cloth::new grid: enough to rebuild its struts, rest lengths and faces on chip.
@@ -69,6 +81,13 @@ hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blo
 // for the CSR kernel that runs after it in the same stream.
 hipError_t launch_faces(const StepArgs& a, const CsrView& t, const BlockRange* face_blocks, uint32_t n_blocks,
                         bool exact, hipStream_t s);
+// Forces of hinges [first, first + count) (SpringyMesh::apply_torsional_forces, springy_mesh.rs:553-627)
+// into t.hinge_f.
+hipError_t launch_hinges(const StepArgs& a, const CsrView& t, const HingeEnt* hinges, uint32_t first, uint32_t count,
+                         bool exact, hipStream_t s);
+// Adds each vertex's hinge forces (summed in strut order from zero) to its force record.
+hipError_t launch_torsion_gather(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks,
+                                 hipStream_t s);
 hipError_t launch_grid(const StepArgs& a, const ClothView* cloths, const BlockRange* blocks, uint32_t n_blocks,
                        int mode, bool exact, hipStream_t s);
 // LDS-tiled grid kernel over `n_tiles` tiles that all have tile shape `shape`.

@@ -74,6 +74,7 @@ enum MeshKernel { MK_CSR = 0, MK_GRID_SIMPLE = 1, MK_GRID_TILED = 2 };
 struct MeshSlot {
   fp::HostMesh host;
   uint64_t vbase = 0, fbase = 0;
+  uint32_t hinge_begin = 0, hinge_end = 0;  // torsional hinges (range in fp_sim::hinges)
   fp_config cfg;
   fp::StepParams params;
   bool uniform_mass = true;
@@ -182,8 +183,18 @@ struct fp_sim {
     a.force_init = force_init ? 1 : 0;
     return a;
   }
-  fp::CsrView csr() const {
-    return fp::CsrView{s_row.p, s_ent.p, f_row.p, f_ent.p, f_v.p, face_sum.p, face_w.p};
+  // torsional hinges of every mesh built with a torsion config (strut order), per vertex their
+  // (hinge << 2 | role) entries, and the per-hinge forces; in the step only when torsion_on
+  DBuf<fp::HingeEnt> hinges;
+  DBuf<uint32_t> t_row, t_ent;
+  DBuf<float4> hinge_f;
+  uint32_t n_hinges = 0;
+  bool torsion_on = false;
+
+  fp::CsrView csr(bool with_torsion = false) const {
+    const bool t = (with_torsion || torsion_on) && n_hinges > 0;
+    return fp::CsrView{s_row.p, s_ent.p, f_row.p, f_ent.p, f_v.p, face_sum.p, face_w.p,
+                       t ? t_row.p : nullptr, t ? t_ent.p : nullptr, hinge_f.p};
   }
 };
 
@@ -438,6 +449,8 @@ int run_steps(fp_sim* s, uint64_t n) {
                                     s->stream));
     HIP_TRY(fp::launch_grid(a, s->cloths.p, s->grid_blocks.p, s->n_grid_blocks, fp::MODE_STEP, exact, s->stream));
     HIP_TRY(fp::launch_faces(a, s->csr(), s->csr_face_blocks.p, s->n_csr_face_blocks, exact, s->stream));
+    if (s->torsion_on)
+      HIP_TRY(fp::launch_hinges(a, s->csr(), s->hinges.p, 0, s->n_hinges, exact, s->stream));
     HIP_TRY(fp::launch_csr(a, s->csr(), s->csr_blocks.p, s->n_csr_blocks, fp::MODE_STEP, exact, s->stream));
     s->cur ^= 1;
     s->forces_dirty = false;
@@ -767,6 +780,50 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     HIP_TRY(s->f_v.upload(fv.data(), fv.size()));
   }
 
+  // torsional hinges (springy_mesh.rs:348-393 builds them; :555-567 names the four vertices)
+  {
+    std::vector<fp::HingeEnt> hs;
+    for (auto& ms_ : s->meshes) {
+      const auto& h = ms_.host;
+      ms_.hinge_begin = (uint32_t)hs.size();
+      for (uint64_t k = 0; k < h.n_struts(); ++k) {
+        if (!h.t_has[k] || h.s_f0[k] < 0 || h.s_f1[k] < 0) continue;
+        const uint32_t a = h.s_v0[k], b = h.s_v1[k];
+        auto other = [&](int64_t f) -> uint32_t {  // utils::tuple_difference(face vertices, strut)
+          for (int j = 0; j < 3; ++j) {
+            const uint32_t v = (uint32_t)h.f_v[3 * (size_t)f + j];
+            if (v != a && v != b) return v;
+          }
+          return a;
+        };
+        fp::HingeEnt e{};
+        e.v[0] = (uint32_t)(ms_.vbase + a);
+        e.v[1] = (uint32_t)(ms_.vbase + b);
+        e.v[2] = (uint32_t)(ms_.vbase + other(h.s_f0[k]));
+        e.v[3] = (uint32_t)(ms_.vbase + other(h.s_f1[k]));
+        e.k = h.t_k[k];
+        e.c = h.t_c[k];
+        e.rest = h.t_rest[k];
+        hs.push_back(e);
+      }
+      ms_.hinge_end = (uint32_t)hs.size();
+    }
+    s->n_hinges = (uint32_t)hs.size();
+    if (!hs.empty()) {
+      std::vector<uint32_t> trow(N + 1, 0);
+      for (const auto& e : hs)
+        for (int r = 0; r < 4; ++r) trow[e.v[r] + 1]++;
+      for (uint64_t i = 0; i < N; ++i) trow[i + 1] += trow[i];
+      std::vector<uint32_t> tent(trow[N]), tfill(trow.begin(), trow.end() - 1);
+      for (uint32_t k = 0; k < hs.size(); ++k)
+        for (uint32_t r = 0; r < 4; ++r) tent[tfill[hs[k].v[r]]++] = (k << 2) | r;
+      HIP_TRY(s->hinges.upload(hs.data(), hs.size()));
+      HIP_TRY(s->t_row.upload(trow.data(), trow.size()));
+      HIP_TRY(s->t_ent.upload(tent.data(), tent.size()));
+      HIP_TRY(s->hinge_f.alloc(4 * hs.size()));
+    }
+  }
+
   // cloth descriptors, block table
   {
     std::vector<fp::ClothView> cv;
@@ -1034,10 +1091,35 @@ int fp_sim_accumulate_forces(fp_sim* s, uint64_t mesh) {
   else {
     HIP_TRY(fp::launch_faces(a, s->csr(), s->all_face_blocks.p + m.fblk_begin, m.fblk_end - m.fblk_begin, exact,
                              s->stream));
+    if (s->torsion_on)
+      HIP_TRY(fp::launch_hinges(a, s->csr(), s->hinges.p, m.hinge_begin, m.hinge_end - m.hinge_begin, exact,
+                                s->stream));
     HIP_TRY(fp::launch_csr(a, s->csr(), blk, nb, fp::MODE_FORCES, exact, s->stream));
   }
   s->forces_dirty = true;
   HIP_TRY(hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_sim_apply_torsional_forces(fp_sim* s, uint64_t mesh) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  const MeshSlot& m = s->meshes[mesh];
+  if (m.hinge_end == m.hinge_begin) return FP_OK;
+  fp::StepArgs a = s->args(true);
+  const bool exact = s->math == FP_MATH_EXACT;
+  HIP_TRY(fp::launch_hinges(a, s->csr(true), s->hinges.p, m.hinge_begin, m.hinge_end - m.hinge_begin, exact,
+                            s->stream));
+  HIP_TRY(fp::launch_torsion_gather(a, s->csr(true), s->all_blocks.p + m.blk_begin, m.blk_end - m.blk_begin,
+                                    s->stream));
+  s->forces_dirty = true;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return FP_OK;
+}
+
+int fp_sim_set_torsion(fp_sim* s, int32_t enabled) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  s->torsion_on = enabled != 0;
+  ++s->epoch;  // captured step graphs no longer match
   return FP_OK;
 }
 

@@ -129,6 +129,33 @@ __device__ __forceinline__ float corner_weight(f3 pa, f3 pnext, f3 pprev) {
   return M::div(th, 3.14159274101257324f);  // Rad / Rad(PI), PI = std::f32::consts::PI
 }
 
+// Forces of one torsional hinge (springy_mesh.rs:555-610, the reference's operation order): hinge
+// strut x0 -> x1, x2 the other vertex of the strut's first face, x3 of its second; velocities of
+// x2 and x3; out[0..3] are the forces on x0, x1, x2, x3.
+template <class M>
+__device__ __forceinline__ void hinge_forces(f3 x0, f3 x1, f3 x2, f3 x3, f3 v2, f3 v3, float k, float c, float rest,
+                                             f3 out[4]) {
+  const f3 d01 = sub(x1, x0), d02 = sub(x2, x0), d03 = sub(x3, x0);
+  const float l01 = M::sqrt_(M::dot(d01, d01));
+  const f3 h = smul(M::rcp(l01), d01);  // normalize: v * (1 / |v|)
+  const float d02h = M::dot(d02, h), d03h = M::dot(d03, h);
+  const f3 rl = sub(d02, smul(d02h, h)), rr = sub(d03, smul(d03h, h));
+  const f3 cl = cross(d01, d02), cr = cross(d03, d01);
+  const f3 nl = smul(M::rcp(M::sqrt_(M::dot(cl, cl))), cl);
+  const f3 nr = smul(M::rcp(M::sqrt_(M::dot(cr, cr))), cr);
+  const float theta = M::atan2_(M::dot(cross(nl, nr), h), M::dot(nl, nr));
+  const float ml = M::sqrt_(M::dot(rl, rl)), mr = M::sqrt_(M::dot(rr, rr));
+  const float tl = M::div(M::dot(v2, nl), ml), tr = M::div(M::dot(v3, nr), mr);
+  const f3 spring = smul(k * (theta - rest), h);
+  const f3 damp = smul((-1.0f * c) * (tl + tr), h);
+  const f3 torque = add(spring, damp);
+  const float th = M::dot(torque, h);
+  out[3] = smul(M::div(th, mr), nr);
+  out[2] = smul(M::div(th, ml), nl);
+  out[1] = sdiv(add(smul(d02h, out[2]), smul(d03h, out[3])), l01);  // Vector3 / f32: per component
+  out[0] = smul(-1.0f, add(add(out[1], out[2]), out[3]));
+}
+
 // The explicit integrators of state.rs:75-106 applied to one Point record (springy_mesh.rs:223-240)
 // with the force frozen over the step (its derivative entries are 0).  RK4 expands to the stage
 // values below; each line is the reference's vec_add/scale of that record component.

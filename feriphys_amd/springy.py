@@ -309,6 +309,15 @@ class Simulation(_Handle):
         rec = np.ascontiguousarray(np.asarray(new_records, np.float32).reshape(-1, 10))
         check(lib().fp_sim_update_points(self.handle, mesh, _fptr(rec), rec.shape[0]))
 
+    def apply_torsional_forces(self, mesh: int = 0):
+        """SpringyMesh::apply_torsional_forces (springy_mesh.rs:553-627) on one mesh's force records."""
+        check(lib().fp_sim_apply_torsional_forces(self.handle, mesh))
+
+    def set_torsion(self, enabled: bool):
+        """Restores the torsional forces the reference comments out of accumulate_forces
+        (springy_mesh.rs:518-519), for every step; off by default, as in the reference."""
+        check(lib().fp_sim_set_torsion(self.handle, 1 if enabled else 0))
+
     def clear_forces(self, mesh: int = 0):
         check(lib().fp_sim_clear_forces(self.handle, mesh))
 

@@ -176,6 +176,15 @@ int fp_sim_accumulate_forces(fp_sim* sim, uint64_t mesh);
 int fp_sim_update_points(fp_sim* sim, uint64_t mesh, const float* new_records, uint64_t n);
 /* SpringyMesh::clear_forces (springy_mesh.rs:656-660). */
 int fp_sim_clear_forces(fp_sim* sim, uint64_t mesh);
+/* SpringyMesh::apply_torsional_forces (springy_mesh.rs:553-627): every strut with a torsional
+ * spring (meshes built with a torsion config) adds its hinge forces to its four vertices; each
+ * vertex's sum, taken in strut order from zero, is added once to its force record. */
+int fp_sim_apply_torsional_forces(fp_sim* sim, uint64_t mesh);
+/* The reference comments apply_torsional_forces out of accumulate_forces (springy_mesh.rs:518-519,
+ * "torsional forces are broken").  enabled = 1 restores that call in its place, between the strut
+ * and the face forces, for every step and fp_sim_accumulate_forces; 0 (the default) is the
+ * reference's behaviour.  Meshes built by Cloth::new have no torsional springs. */
+int fp_sim_set_torsion(fp_sim* sim, int32_t enabled);
 
 /* Kernel selection.  FP_KERNEL_AUTO uses the LDS-tiled grid kernel for meshes built by Cloth::new
  * and left unchanged, the CSR kernel otherwise; FP_KERNEL_GRID_SIMPLE is the one-thread-per-vertex

@@ -160,6 +160,7 @@ template <class R> struct Sim {
   int orc_mesh_apply_torsional_forces##SFX(void* h) {                                                                \
     try { static_cast<SpringyMesh<R>*>(h)->apply_torsional_forces(); return 0; } catch (...) { return 1; }          \
   }                                                                                                                  \
+  void orc_mesh_set_torsion_in_step##SFX(void* h, int on) { static_cast<SpringyMesh<R>*>(h)->torsion_in_step = on != 0; } \
   void orc_mesh_clear_forces##SFX(void* h) { static_cast<SpringyMesh<R>*>(h)->clear_forces(); }                      \
   int orc_mesh_update_points##SFX(void* h, const R* new10, size_t n, void** obs, size_t nobs,                        \
                                   const CConfig<R>* cfg) {                                                           \

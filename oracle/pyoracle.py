@@ -72,6 +72,7 @@ def _declare(L):
             "orc_mesh_apply_strut_forces": (None, [_P]),
             "orc_mesh_apply_face_forces": (None, [_P, CP]),
             "orc_mesh_apply_torsional_forces": (C.c_int, [_P]),
+            "orc_mesh_set_torsion_in_step": (None, [_P, C.c_int]),
             "orc_mesh_clear_forces": (None, [_P]),
             "orc_mesh_update_points": (C.c_int, [_P, RP, C.c_size_t, C.POINTER(_P), C.c_size_t, CP]),
             "orc_mesh_integrate": (C.c_int, [_P, CP, RP]),
@@ -255,6 +256,11 @@ class Mesh:
     def apply_face_forces(self, cfg: Config):
         c = cfg.c(self.dtype)
         _fn("orc_mesh_apply_face_forces", self.dtype)(self.h, C.byref(c))
+
+    def set_torsion_in_step(self, on: bool):
+        """Restores the apply_torsional_forces call the reference comments out of accumulate_forces
+        (springy_mesh.rs:518-519), between the strut and the face forces."""
+        _fn("orc_mesh_set_torsion_in_step", self.dtype)(self.h, 1 if on else 0)
 
     def apply_torsional_forces(self):
         if _fn("orc_mesh_apply_torsional_forces", self.dtype)(self.h):

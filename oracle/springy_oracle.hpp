@@ -455,9 +455,14 @@ template <class R> struct SpringyMesh {
   }
 
   // :515-525
+  // The reference comments out apply_torsional_forces at :518-519; torsion_in_step restores that
+  // call in its place (between the strut and the face forces).
+  bool torsion_in_step = false;
+
   void accumulate_forces(const Config<R>& cfg) {
     for (auto& p : points) p.accumulated_force += cfg.gravity;  // :527-531, :194-196
     apply_strut_forces();
+    if (torsion_in_step) apply_torsional_forces();
     apply_face_forces(cfg);
     for (size_t pi : pinned_points) {
       if (pi >= points.size()) throw std::out_of_range("pin index");
