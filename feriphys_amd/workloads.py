@@ -225,6 +225,20 @@ def config2(n_cloths: int = 4096, seed: int = 0, smin: int = 8, smax: int = 256,
     return sc
 
 
+def springy_cube():
+    """The reference's springy cube demo (demos/spring_mass_damper.rs:230-272, graphics/forms.rs:91-146,
+    :329-342): a unit cube of 8 vertices and 12 triangles, total mass 80, struts at the stiff
+    defaults (999999, 700, springy_mesh.rs:12-13) plus six diagonals by add_strut, torsional springs
+    configured (7000, 7000; :16-17) but not applied in the step; a 2 x 2 floor square at y = -2.
+    Returns (positions, indices, mass, k, c, torsion, extra struts, floor positions, floor indices)."""
+    pos = np.array([[-0.5, -0.5, 0.5], [0.5, -0.5, 0.5], [0.5, 0.5, 0.5], [-0.5, 0.5, 0.5],
+                    [-0.5, -0.5, -0.5], [0.5, -0.5, -0.5], [0.5, 0.5, -0.5], [-0.5, 0.5, -0.5]], np.float32)
+    idx = [0, 1, 2, 2, 3, 0, 1, 5, 6, 6, 2, 1, 7, 6, 5, 5, 4, 7, 4, 0, 3, 3, 7, 4, 4, 5, 1, 1, 0, 4, 3, 2, 6, 6, 7, 3]
+    extra = [(1, 3), (2, 5), (4, 6), (0, 7), (0, 5), (2, 7)]
+    floor = np.array([[-1.0, -2.0, 1.0], [1.0, -2.0, 1.0], [1.0, -2.0, -1.0], [-1.0, -2.0, -1.0]], np.float32)
+    return pos, idx, 80.0, 999999.0, 700.0, (7000.0, 7000.0), extra, floor, [0, 1, 2, 0, 2, 3]
+
+
 def surrogate_training_scene(n_cloths: int = 128, seed: int = 0, smin: int = 12, smax: int = 48) -> Scene:
     """Cloths whose plain trajectories train the surrogate: rows and cols each log-uniform in
     [smin, smax], one full edge pinned (chosen at random), config-0 stiffness, mass, spacing,
