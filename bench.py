@@ -301,6 +301,34 @@ def bench_surfaces(args, np, fa, wl):
         cpu = {"value": vsps / N * dt, "unit": "sim-s/s", "cores": 1, "kind": "port",
                "sample": f"{args.cpu_steps} steps of 8 icospheres ({n_sub} vertices), scaled by vertex count to "
                          f"{N} vertices", "vertex_steps_per_s": vsps}
+    sur = None
+    wpath = os.path.join(ROOT, "feriphys_amd", "weights", "surface_surrogate.npy")
+    if args.weights and os.path.exists(wpath):
+        # the same scene with the irregular-mesh surrogate (8x steps, per-mesh fallback), against the
+        # plain run's final positions (the simulation above ran exactly args.steps plain steps)
+        ref = [sim.get_points(k)[:, 1:4].astype(np.float64) for k in range(len(scene.meshes))]
+        net = fa.Surrogate(fa.SurrogateConfig(), params=np.load(wpath, allow_pickle=False))
+        ssim = wl.build_surface_gpu(scene)
+        ssim.set_surrogate(net)
+        n_big = args.steps // net.config.ratio
+        if args.warmup:
+            ssim.step_surrogate(max(1, args.warmup // net.config.ratio))
+            for k, rec in enumerate(initial):
+                ssim.set_points(k, rec)
+        total_ms, mlp_ms, st = ssim.time_surrogate(n_big)
+        err2, ext = 0.0, max(float(np.ptp(m.positions, axis=0).max()) for m in scene.meshes)
+        for k in range(len(scene.meshes)):
+            d = ssim.get_points(k)[:, 1:4].astype(np.float64) - ref[k]
+            err2 += float(np.sum(d * d))
+        launches = n_big - st.plain_only
+        sur = {"sim_s_per_s": n_big * net.config.ratio * dt / (total_ms * 1e-3),
+               "speedup_vs_plain": (n_big * net.config.ratio * dt / (total_ms * 1e-3)) / (args.steps * dt / wall),
+               "big_steps": n_big, "fallbacks": st.fallbacks, "mesh_fallbacks": st.mesh_fallbacks,
+               "plain_only": st.plain_only, "network_launches": launches,
+               "rms_error_over_extent": float(np.sqrt(err2 / N)) / ext,
+               "error_definition": "final positions vs the plain run after the same simulated time; extent = "
+                                   f"the largest mesh's diameter {ext:.3f} m",
+               "mlp_kernel_avg_us": mlp_ms * 1e3 / max(1, launches)}
     line = {
         "metric": "sim-s/s, irregular meshes (64 icospheres, CSR); %HBM peak",
         "value": args.steps * dt / wall, "unit": "sim-s/s", "n_gpus": 1, "steps": args.steps,
@@ -313,6 +341,7 @@ def bench_surfaces(args, np, fa, wl):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_avg_us": kernel_avg_s * 1e6,
                      "bytes_per_launch": bytes_step},
         "cpu_baseline": cpu,
+        "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
 

@@ -4,10 +4,11 @@
 
 Data: the plain solver's trajectories (600 steps at dt = 1/600 s) of seeded training cloths
 (workloads.surrogate_training_scene: one pinned edge, random wind; 2,000 with sides in [12, 96]
-and a few with sides in [192, 512]), sampled every 8 steps, with normal position noise on the
-inputs.  Training: Adam on the library's bf16 MFMA kernels, in three
-learning-rate phases.  Evaluation: the 32 x 32 edge-pinned cloth stepped 75 big steps with the
-fallback off, against 600 plain steps.
+and a few with sides in [192, 512]) or, with --scene surfaces, icospheres dropped on a ground
+plane (workloads.surface_training_scene), sampled every 8 steps, with normal position noise on
+the inputs.  Training: Adam on the library's bf16 MFMA kernels, in three
+learning-rate phases.  Evaluation: the 32 x 32 edge-pinned cloth (or six config4 icospheres)
+stepped 75 big steps with the fallback off, against 600 plain steps.
 """
 from __future__ import annotations
 
@@ -23,6 +24,25 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 import feriphys_amd as fa  # noqa: E402
 from feriphys_amd import workloads as wl  # noqa: E402
+
+
+def evaluate_surfaces(net, big_steps=75):
+    """Six icospheres of the config4 recipe (levels 4-5) on the ground plane, 75 big steps with the
+    fallback off against 600 plain steps; error relative to the largest mesh's diameter."""
+    sc = wl.config4(n_meshes=6, seed=5, levels=[5, 5, 4, 4, 4, 4])
+    ref, sur = wl.build_surface_gpu(sc), wl.build_surface_gpu(sc)
+    off = fa.Surrogate(fa.SurrogateConfig(strain_limit=1e30, energy_limit=1e30), params=net.get_params())
+    sur.set_surrogate(off)
+    extent = max(float(np.ptp(m.positions, axis=0).max()) for m in sc.meshes)
+    out = []
+    for k in range(1, big_steps + 1):
+        ref.step(8)
+        sur.step_surrogate(1)
+        if k in (1, 2, 5, 10, 20, 40, 75):
+            d = np.concatenate([sur.get_points(m)[:, 1:4].astype(np.float64) - ref.get_points(m)[:, 1:4]
+                                for m in range(len(sc.meshes))])
+            out.append((k, float(np.sqrt(np.mean(np.sum(d * d, axis=1))) / extent)))
+    return out
 
 
 def evaluate(net, big_steps=75, seed=21):
@@ -54,20 +74,31 @@ def main():
     ap.add_argument("--groups", type=int, default=4, help="trajectories are simulated in this many batches")
     ap.add_argument("--large", type=int, default=16,
                     help="extra cloths with sides in [192, 512] (the tension of long hanging cloths)")
+    ap.add_argument("--scene", choices=["cloths", "surfaces"], default="cloths",
+                    help="cloths: grid cloths (the 1024 x 1024 cloth's network); surfaces: icospheres (config4's)")
+    ap.add_argument("--meshes", type=int, default=400, help="icospheres per training scene (--scene surfaces)")
     ap.add_argument("--out", default="feriphys_amd/weights/cloth_surrogate.npy")
     args = ap.parse_args()
 
     t0 = time.time()
     net = fa.Surrogate(seed=args.seed)
     per = (args.cloths + args.groups - 1) // args.groups
-    for g in range(args.groups):
+    for g in range(args.groups if args.scene == "cloths" else 0):
         n = min(per, args.cloths - g * per)
         sc = wl.surrogate_training_scene(n_cloths=n, seed=1000 * (args.seed + 1) + g, smin=args.smin, smax=args.smax)
         sim = wl.build_gpu(sc)
         net.collect(sim, n_big=args.big, noise=args.noise, seed=args.seed * 7919 + g)
         del sim
         print(f"group {g}: {n} cloths, {net.sample_count()} samples so far ({time.time() - t0:.1f} s)", flush=True)
-    if args.large:
+    if args.scene == "surfaces":
+        for g in range(args.groups):
+            sc = wl.surface_training_scene(n_meshes=args.meshes, seed=1000 * (args.seed + 1) + g)
+            sim = wl.build_surface_gpu(sc)
+            net.collect(sim, n_big=args.big, noise=args.noise, seed=args.seed * 7919 + g)
+            del sim
+            print(f"group {g}: {args.meshes} icospheres, {net.sample_count()} samples so far "
+                  f"({time.time() - t0:.1f} s)", flush=True)
+    if args.large and args.scene == "cloths":
         sc = wl.surrogate_training_scene(n_cloths=args.large, seed=7777 + args.seed, smin=192, smax=512)
         sim = wl.build_gpu(sc)
         net.collect(sim, n_big=args.big, noise=args.noise, seed=args.seed * 7919 + 99)
@@ -85,7 +116,7 @@ def main():
         print(f"phase {i}: {steps} steps at lr {lr}: loss {log[-1]['loss_first']:.4f} -> {log[-1]['loss_last']:.4f} "
               f"({time.time() - t1:.1f} s)", flush=True)
     train_s = time.time() - t1
-    ev = evaluate(net)
+    ev = evaluate(net) if args.scene == "cloths" else evaluate_surfaces(net)
     for k, e in ev:
         print(f"eval big step {k}: rms position error / extent {e:.2e}", flush=True)
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)

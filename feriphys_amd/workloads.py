@@ -356,6 +356,34 @@ def config4(n_meshes: int = 64, seed: int = 0, levels=None) -> "SurfaceScene":
     return SurfaceScene(meshes, cfg, [ground], f"{len(meshes)} icospheres")
 
 
+def surface_training_scene(n_meshes: int = 400, seed: int = 0, levels=(2, 3, 4, 5)) -> "SurfaceScene":
+    """Icospheres whose plain trajectories train the irregular-mesh surrogate: the config4 recipe
+    (radial +-2 % noise, edge and bend struts per unit strain at the mean edge length, 0.005 kg per
+    vertex, ground plane at y = 0) with levels drawn from `levels`, drop heights 0.05-0.5 m and a
+    per-scene wind of the config4 direction scaled by a factor in [0, 3] (the same wind for all
+    meshes: one Config)."""
+    rng = np.random.default_rng(seed)
+    meshes, cache = [], {}
+    for k in range(n_meshes):
+        lv = int(rng.choice(levels))
+        if lv not in cache:
+            cache[lv] = icosphere_fast(lv)
+        v, f = cache[lv]
+        r = 0.004 * 2 ** lv * rng.uniform(0.8, 1.25)
+        noise = rng.uniform(-0.02, 0.02, size=(v.shape[0], 1))
+        centre = np.array([(k % 20) * 3.0, r + rng.uniform(0.05, 0.5), (k // 20) * 3.0])
+        pos = (v * r * (1.0 + noise) + centre).astype(F32)
+        e = np.linalg.norm(pos[f[:, 0]].astype(np.float64) - pos[f[:, 1]], axis=1).mean()
+        meshes.append(SurfaceSpec(pos, f.reshape(-1).astype(np.uint64), float(F32(0.005 * v.shape[0])),
+                                  per_strain(100.0, e), per_strain(0.3, e), per_strain(10.0, 2 * e),
+                                  per_strain(0.05, 2 * e)))
+    ground = (np.array([[-1e3, 0.0, 1e3], [1e3, 0.0, 1e3], [1e3, 0.0, -1e3], [-1e3, 0.0, -1e3]], F32),
+              np.array([0, 1, 2, 0, 2, 3], np.uint64))
+    w = float(rng.uniform(0.0, 3.0))
+    cfg = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.005), (w, 0.0, 0.5 * w), 0.2, 0.5, 0.3, 0.2).f32()
+    return SurfaceScene(meshes, cfg, [ground], f"{n_meshes} training icospheres")
+
+
 @dataclass
 class SurfaceScene:
     meshes: List[SurfaceSpec]
