@@ -96,6 +96,26 @@ def main():
     kernel_avg_s = kernel_ms * 1e-3 / args.steps
     achieved = BYTES_PER_VERTEX_STEP * N / kernel_avg_s / 1e9
 
+    # the same 600 steps with the fast math policy (one-step forces within 1e-5 of the exact ones;
+    # 600-step trajectories checked against the f64 oracle in the parity tests)
+    fast = None
+    if args.math == "exact":
+        sim.set_kernel(kern, fa.FP_MATH_FAST)
+        sim.set_points(0, initial)
+        sim.step(max(1, args.warmup))
+        sim.set_points(0, initial)
+        sim.synchronize()
+        tf = time.perf_counter()
+        fast_ms = sim.time_steps(args.steps)
+        sim.synchronize()
+        fwall = time.perf_counter() - tf
+        xf = sim.get_points(0)[:, 1:4].astype(np.float64)
+        favg = fast_ms * 1e-3 / args.steps
+        fast = {"sim_s_per_s": args.steps * dt / fwall, "kernel_avg_us": favg * 1e6,
+                "hbm_frac": BYTES_PER_VERTEX_STEP * N / favg / 1e9 / HBM_PEAK_GBS,
+                "max_position_difference_vs_exact_m": float(np.abs(xf - pts[:, 1:4]).max())}
+        sim.set_kernel(kern, fa.FP_MATH_EXACT)
+
     sur = surrogate_leg(args, np, fa, wl, kern, pts, initial, N, dt, sim_s_per_s)
 
     cpu = None
@@ -140,6 +160,7 @@ def main():
                                f"320-face icosphere collider, RK4 dt=1/600 s",
                    "vertices": N, "dt": dt, "kernel": args.kernel, "math": args.math},
         "vertex_steps_per_s": args.steps * N / wall,
+        "fast_math": fast,
         "surrogate": sur,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
@@ -242,6 +263,44 @@ def bench_batched(args, np, fa, wl):
                "sample": f"{args.cpu_steps} steps of the first 64 cloths ({n_sub} vertices), scaled by vertex count "
                          f"to the {N}-vertex batch (single-threaded C++ restatement of the reference step, one "
                          f"Simulation per cloth)", "vertex_steps_per_s": vsps}
+    # the same cloths with the cloth surrogate network (8x steps, per-mesh fallback): a rejection
+    # costs only its own mesh's plain steps
+    sur = None
+    if args.weights and os.path.exists(args.weights):
+        M = len(scene.cloths)
+        ref = [sim.get_points(k)[:, 1:4].astype(np.float64) for k in range(M)]
+        net = fa.Surrogate(fa.SurrogateConfig(), params=np.load(args.weights, allow_pickle=False))
+        ssim = wl.build_gpu(scene)
+        ssim.set_surrogate(net)
+        ratio = net.config.ratio
+        n_big = args.steps // ratio
+        if args.warmup:
+            ssim.step_surrogate(max(1, args.warmup // ratio))
+            for k, rec in enumerate(initial):
+                ssim.set_points(k, rec)
+        total_ms, mlp_ms, st = ssim.time_surrogate(n_big)
+        rel, diverged = [], 0
+        for k, c in enumerate(scene.cloths):
+            if not (np.isfinite(ref[k]).all() and np.abs(ref[k]).max() < 1e3):
+                diverged += 1  # the plain method itself blows up on this cloth (e.g. corner pins)
+                continue
+            d = ssim.get_points(k)[:, 1:4].astype(np.float64) - ref[k]
+            rel.append(float(np.sqrt(np.mean(np.sum(d * d, axis=1)))) / ((max(c.rows, c.cols) - 1) * c.spacing))
+        rel = np.array(rel)
+        rel = np.where(np.isfinite(rel), rel, np.inf)
+        sim_s = n_big * ratio * dt / (total_ms * 1e-3)
+        launches = n_big - st.plain_only
+        sur = {"sim_s_per_s": sim_s, "speedup_vs_plain": sim_s / (args.steps * dt / wall), "big_steps": n_big,
+               "fallbacks": st.fallbacks, "mesh_fallbacks": st.mesh_fallbacks,
+               "accepted_mesh_steps_frac": 1.0 - st.mesh_fallbacks / float(M * n_big),
+               "plain_only": st.plain_only, "network_launches": launches,
+               "rms_error_over_extent_median": float(np.median(rel)) if rel.size else None,
+               "rms_error_over_extent_p90": float(np.percentile(rel, 90)) if rel.size else None,
+               "cloths_within_1e-3": int(np.sum(rel <= 1e-3)), "cloths_compared": int(rel.size),
+               "cloths_diverging_in_plain_method": diverged, "cloths": M,
+               "error_definition": "per cloth: RMS of final-position differences vs the plain run after the same "
+                                   "simulated time, over the cloth's longer side",
+               "mlp_kernel_avg_us": mlp_ms * 1e3 / max(1, launches)}
     line = {
         "metric": "sim-s/s, batched cloths (4096 meshes, per-mesh Config); %HBM peak",
         "value": args.steps * dt / wall,
@@ -257,6 +316,7 @@ def bench_batched(args, np, fa, wl):
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_avg_us": kernel_avg_s * 1e6,
                      "bytes_per_launch": BYTES_PER_VERTEX_STEP * N},
         "cpu_baseline": cpu,
+        "surrogate": sur,
     }
     print(json.dumps(line), flush=True)
 
