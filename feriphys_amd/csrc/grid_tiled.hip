@@ -111,10 +111,12 @@ __global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const Cl
     const uint32_t i = G.base + (uint32_t)(r * C + c);
     const f3 p = Pl(r, c), v = Vl(r, c);
     const float m = P.uniform_mass ? P.mass : A.mass[i];
-    f3 F = A.force_init ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
-    F = add(F, P.gravity);
+    // MODE_SURROGATE: F collects only the face forces (from zero); gravity and the network's
+    // acceleration enter in sur_update
+    f3 F = (MODE != MODE_SURROGATE && A.force_init) ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
+    if (MODE != MODE_SURROGATE) F = add(F, P.gravity);
     auto strut = [&](int rr, int cc, const StrutK& K) {
-      strut_accumulate<M>(F, p, v, Pl(rr, cc), Vl(rr, cc), K.k, K.c, K.L);
+      if (MODE != MODE_SURROGATE) strut_accumulate<M>(F, p, v, Pl(rr, cc), Vl(rr, cc), K.k, K.c, K.L);
     };
     const int qi = r - r0, qj = c - c0;  // quad (r-1, c-1) is s_*[qi][qj]
     // struts in strut-index order (see grid_kernel in kernels.hip)
@@ -140,6 +142,17 @@ __global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const Cl
     if (r >= 1 && c <= C - 2) face(1, qi, qj + 1, corner_weight<M>(p, Pl(r - 1, c + 1), Pl(r, c + 1)));
     if (r <= R - 2 && c >= 1) face(1, qi + 1, qj, corner_weight<M>(p, Pl(r + 1, c), Pl(r + 1, c - 1)));
     const bool pin = pinned(A.pins, i);
+    if (MODE == MODE_SURROGATE) {
+      f3 xn = p, vn = v;
+      float w = 0.0f;
+      if (!pin)
+        w = sur_update(P, A.obs, m, p, v, F, f3{0.0f, 0.0f, 0.0f},
+                       f3{A.da[3 * (size_t)i], A.da[3 * (size_t)i + 1], A.da[3 * (size_t)i + 2]}, xn, vn);
+      A.out.x[i] = xn.x; A.out.y[i] = xn.y; A.out.z[i] = xn.z;
+      A.out.vx[i] = vn.x; A.out.vy[i] = vn.y; A.out.vz[i] = vn.z;
+      A.work[i] = w;
+      continue;
+    }
     if (pin) F = f3{0.0f, 0.0f, 0.0f};
     if (MODE == MODE_FORCES) {
       A.fx[i] = F.x; A.fy[i] = F.y; A.fz[i] = F.z;
@@ -160,7 +173,8 @@ template <class M, int TR, int TC>
 hipError_t launch_shape(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n, int mode,
                         hipStream_t s) {
   if (mode == MODE_STEP) grid_tiled_kernel<M, TR, TC, MODE_STEP><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
-  else grid_tiled_kernel<M, TR, TC, MODE_FORCES><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
+  else if (mode == MODE_FORCES) grid_tiled_kernel<M, TR, TC, MODE_FORCES><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
+  else grid_tiled_kernel<M, TR, TC, MODE_SURROGATE><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
   return hipGetLastError();
 }
 

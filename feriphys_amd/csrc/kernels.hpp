@@ -58,9 +58,13 @@ struct StepArgs {
   const StepParams* params;
   ObsView obs;
   int32_t force_init;   // 1: start from the force records and clear them after the step
+  const float* da;      // surrogate mode: the network's acceleration, 3 per vertex
+  float* work;          // surrogate mode: each vertex's face-force work over the step
 };
 
-enum StepMode { MODE_STEP = 0, MODE_FORCES = 1 };
+// MODE_SURROGATE: the learned big step (gravity + face forces, the network's acceleration,
+// position-difference update; see sur_update in step_device.hpp).
+enum StepMode { MODE_STEP = 0, MODE_FORCES = 1, MODE_SURROGATE = 2 };
 
 // One workgroup's tile of a Cloth::new grid for the LDS-tiled kernel: rows [r0, r0+TR) and columns
 // [c0, c0+TC) of mesh `mesh` (clipped to the grid).

@@ -119,6 +119,8 @@ struct fp_sim {
   fp::ObsView obs{};
   uint64_t n_obs_vertices = 0, n_obs_faces = 0;
   DBuf<fp::BlockRange> all_blocks, grid_blocks, csr_blocks;
+  DBuf<fp::BlockRange> sur_blocks, sur_face_blocks;  // meshes off the tiled kernel (surrogate step)
+  uint32_t n_sur_blocks = 0, n_sur_face_blocks = 0;
   std::vector<fp::BlockRange> h_blocks;
   uint32_t n_grid_blocks = 0, n_csr_blocks = 0;
   // LDS-tiled grid kernel: per mesh its tiles (all of the mesh's tile shape), and per shape the
@@ -269,7 +271,7 @@ std::vector<fp::TileRange> xcd_order(const std::vector<fp::TileRange>& t) {
 }
 
 int build_block_lists(fp_sim* s) {
-  std::vector<fp::BlockRange> g, c, cf;
+  std::vector<fp::BlockRange> g, c, cf, sb, sf;
   std::vector<fp::TileRange> by_shape[fp::N_TILE_SHAPES], all_tiles;
   for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
     MeshSlot& ms = s->meshes[mi];
@@ -286,10 +288,21 @@ int build_block_lists(fp_sim* s) {
       ms.tile_end = (uint32_t)all_tiles.size();
       continue;
     }
-    for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
-    if (ms.kernel == MK_CSR)
-      for (uint32_t b = ms.fblk_begin; b < ms.fblk_end; ++b) cf.push_back(s->h_face_blocks[b]);
+    for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) {
+      (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
+      sb.push_back(s->h_blocks[b]);
+    }
+    for (uint32_t b = ms.fblk_begin; b < ms.fblk_end; ++b) {
+      if (ms.kernel == MK_CSR) cf.push_back(s->h_face_blocks[b]);
+      sf.push_back(s->h_face_blocks[b]);
+    }
   }
+  // the surrogate's generic path (face kernel + per-vertex update) covers every mesh that is not
+  // on the tiled kernel
+  s->n_sur_blocks = (uint32_t)sb.size();
+  s->n_sur_face_blocks = (uint32_t)sf.size();
+  HIP_TRY(s->sur_blocks.upload(sb.data(), sb.size()));
+  HIP_TRY(s->sur_face_blocks.upload(sf.data(), sf.size()));
   s->n_csr_face_blocks = (uint32_t)cf.size();
   HIP_TRY(s->csr_face_blocks.upload(cf.data(), cf.size()));
   s->n_grid_blocks = (uint32_t)g.size();
@@ -1295,13 +1308,26 @@ int sur_big_step(fp_sim* s, fp_surrogate_stats& st, float* mlp_ms) {
   }
   fp::StepArgs a = s->args(false);
   a.params = S.big_params.p;
-  HIP_TRY(fp::launch_faces(a, s->csr(), s->all_face_blocks.p, (uint32_t)s->h_face_blocks.size(), false, s->stream));
+  a.da = S.da.p;
+  a.work = S.work.p;
+  // meshes off the tiled kernel: face kernel now, per-vertex update after the network; cloths on
+  // the tiled kernel take their faces and the update in one LDS-tiled pass (same arithmetic)
+  // (with base_struts every mesh takes the generic path: the tiled pass has no strut forces)
+  const bool fused = S.cfg.base_struts == 0;
+  const fp::BlockRange* fb = fused ? s->sur_face_blocks.p : s->all_face_blocks.p;
+  const uint32_t nfb = fused ? s->n_sur_face_blocks : (uint32_t)s->h_face_blocks.size();
+  const fp::BlockRange* vb = fused ? s->sur_blocks.p : s->all_blocks.p;
+  const uint32_t nvb = fused ? s->n_sur_blocks : nblk;
+  HIP_TRY(fp::launch_faces(a, s->csr(), fb, nfb, false, s->stream));
   if (mlp_ms) HIP_TRY(hipEventRecord(s->ev0, s->stream));
   HIP_TRY(fp::launch_sur_mlp(a.in, S.slots.p, (uint32_t)s->N, S.image.p, feat_params(S.cfg), S.cfg.out_scale, S.da.p,
                              S.grid, S.variant, s->stream));
   if (mlp_ms) HIP_TRY(hipEventRecord(s->ev1, s->stream));
-  HIP_TRY(fp::launch_sur_integrate(a, s->csr(), s->all_blocks.p, nblk, S.da.p, S.work.p, S.cfg.base_struts != 0,
-                                   s->stream));
+  HIP_TRY(fp::launch_sur_integrate(a, s->csr(), vb, nvb, S.da.p, S.work.p, !fused, s->stream));
+  if (fused)
+    for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh)
+      HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->tiles[sh].p, s->n_tiles[sh], sh, fp::MODE_SURROGATE, false,
+                                    s->stream));
   HIP_TRY(fp::launch_sur_diag(a.in, a.out, S.slots.p, S.slot_k.p, S.big_params.p, s->mass.p, S.work.p, s->all_blocks.p,
                               nblk, S.diag.p, s->stream));
   std::vector<fp::DiagOut> d(nblk);

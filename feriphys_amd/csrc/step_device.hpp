@@ -286,6 +286,24 @@ __device__ __forceinline__ void collide_response(const ObsFace& f, const StepPar
   vn = add(vrn, vrt);
 }
 
+// The learned surrogate's big step for one free vertex (every kernel that takes it calls this, with
+// explicit fused multiply-adds, so that its results do not depend on a file's contraction setting):
+// a = ((gravity + Ff) + Fx) / m + da, v' = v + H a, x' = x + H v', then the collision response over
+// H with the force m a; returns the face-force work Ff . (x' - x).  Fx: explicit strut forces when
+// the big step keeps them (their energy is counted as spring energy, not as work).
+__device__ __forceinline__ float sur_update(const StepParams& P, const ObsView& O, float m, f3 p, f3 v, f3 Ff,
+                                            f3 Fx, f3 da, f3& xn, f3& vn) {
+  const f3 F = add(add(P.gravity, Ff), Fx);
+  const float im = 1.0f / m;
+  const f3 a = f3{__builtin_fmaf(F.x, im, da.x), __builtin_fmaf(F.y, im, da.y), __builtin_fmaf(F.z, im, da.z)};
+  const float H = P.dt;
+  vn = f3{__builtin_fmaf(H, a.x, v.x), __builtin_fmaf(H, a.y, v.y), __builtin_fmaf(H, a.z, v.z)};
+  xn = f3{__builtin_fmaf(H, vn.x, p.x), __builtin_fmaf(H, vn.y, p.y), __builtin_fmaf(H, vn.z, p.z)};
+  const int32_t hit = first_hit(O, p, xn, P.dt_d);
+  if (hit >= 0) collide_response(O.faces[hit], P, m, p, v, f3{a.x * m, a.y * m, a.z * m}, xn, vn);
+  return __builtin_fmaf(Ff.z, xn.z - p.z, __builtin_fmaf(Ff.y, xn.y - p.y, Ff.x * (xn.x - p.x)));
+}
+
 __device__ __forceinline__ bool pinned(const uint32_t* __restrict__ bits, uint32_t i) {
   return bits != nullptr && ((bits[i >> 5] >> (i & 31u)) & 1u);
 }

@@ -224,21 +224,18 @@ __global__ void __launch_bounds__(kBlock) sur_integrate_kernel(StepArgs A, CsrVi
     const float w = T.face_w[3 * (ent >> 2) + (ent & 3u)];
     Ff = add(Ff, f3{w * sum.x, w * sum.y, w * sum.z});
   }
-  f3 F = add(P.gravity, Ff);
+  f3 Fs = f3{0.0f, 0.0f, 0.0f};
   if (STRUTS)
     for (uint32_t e = T.s_row[i], ee = T.s_row[i + 1]; e < ee; ++e) {
       const StrutEnt se = T.s_ent[e];
-      strut_accumulate<FastMath>(F, p, v, ld3(S.x, S.y, S.z, se.nbr), ld3(S.vx, S.vy, S.vz, se.nbr), se.k, se.c, se.len);
+      strut_accumulate<FastMath>(Fs, p, v, ld3(S.x, S.y, S.z, se.nbr), ld3(S.vx, S.vy, S.vz, se.nbr), se.k, se.c,
+                                 se.len);
     }
-  const float im = 1.0f / m;
-  const f3 a = f3{F.x * im + da[3 * (size_t)i], F.y * im + da[3 * (size_t)i + 1], F.z * im + da[3 * (size_t)i + 2]};
-  const float H = P.dt;
-  f3 vn = f3{v.x + H * a.x, v.y + H * a.y, v.z + H * a.z};
-  f3 xn = f3{p.x + H * vn.x, p.y + H * vn.y, p.z + H * vn.z};
-  const int32_t hit = first_hit(A.obs, p, xn, P.dt_d);
-  if (hit >= 0) collide_response(A.obs.faces[hit], P, m, p, v, f3{a.x * m, a.y * m, a.z * m}, xn, vn);
+  const f3 d = f3{da[3 * (size_t)i], da[3 * (size_t)i + 1], da[3 * (size_t)i + 2]};
+  f3 xn, vn;
+  const float w = sur_update(P, A.obs, m, p, v, Ff, Fs, d, xn, vn);
   store_state(A.out, i, xn, vn);
-  work[i] = Ff.x * (xn.x - p.x) + Ff.y * (xn.y - p.y) + Ff.z * (xn.z - p.z);
+  work[i] = w;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -199,3 +199,21 @@ def test_trained_network_gains_over_the_big_step_without_it(trained):
     es, eb = out[-1][1], out[-1][2]
     assert np.isfinite(es) and es < 5e-2
     assert not np.isfinite(eb) or eb > 10 * es
+
+
+def test_tiled_surrogate_pass_equals_generic_path():
+    """Cloths on the LDS-tiled kernel take the big step's faces and update in one tiled pass; the
+    same meshes forced onto the CSR path take the face kernel and the per-vertex update.  Same
+    operations, same order: identical bits, collisions with a sphere included."""
+    sc = wl.config1(n=64, steps=1)
+    params = fa.Surrogate(seed=4).get_params() * 0.3
+    cfg = fa.SurrogateConfig(strain_limit=1e30, energy_limit=1e30)
+    tiled, generic = wl.build_gpu(sc), wl.build_gpu(sc)
+    generic.set_kernel(fa.FP_KERNEL_CSR, fa.FP_MATH_EXACT)
+    for g in (tiled, generic):
+        g.set_surrogate(fa.Surrogate(cfg, params=params))
+    for k in range(6):
+        a, b = tiled.step_surrogate(1), generic.step_surrogate(1)
+        assert a.fallbacks == b.fallbacks == 0
+        assert (bits(tiled.get_points(0)) == bits(generic.get_points(0))).all(), f"big step {k + 1}"
+    assert np.isfinite(tiled.get_points(0)).all()
