@@ -78,24 +78,17 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
       F = add(F, ts);
     }
   }
-  // apply_face_forces (:629-654): the face kernel has computed each face's drag + lift and corner
-  // weights; this vertex adds weight * force for its faces in face order
+  // apply_face_forces (:629-654): the face kernel has stored each corner's weight * (drag + lift)
+  // in this vertex's run of f_row entries; they are added in face order
   {
     const uint32_t e0 = T.f_row[i], ee = T.f_row[i + 1];
     for (uint32_t e = e0; e < ee; e += 4) {
-      uint32_t ent[4];
-      float4 sum[4];
-      float w[4];
+      float4 fc[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) ent[j] = T.f_ent[e + j < ee ? e + j : e];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        sum[j] = T.face_sum[ent[j] >> 2];
-        w[j] = T.face_w[3 * (ent[j] >> 2) + (ent[j] & 3u)];
-      }
+      for (int j = 0; j < 4; ++j) fc[j] = T.face_c[e + j < ee ? e + j : e];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (e + j < ee) F = add(F, f3{w[j] * sum[j].x, w[j] * sum[j].y, w[j] * sum[j].z});
+        if (e + j < ee) F = add(F, f3{fc[j].x, fc[j].y, fc[j].z});
     }
   }
   const bool pin = pinned(A.pins, i);
@@ -108,7 +101,8 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
 }
 
 // Drag + lift (springy_mesh.rs:631-646) and the three corner weights angle / PI (:647-649) of every
-// face of the CSR meshes, once per face.
+// face of the CSR meshes, once per face; corner k's share weight_k * (drag + lift) (the product of
+// :650-652) goes to its vertex's run of f_row entries.
 template <class M>
 __global__ void __launch_bounds__(kBlock) face_kernel(StepArgs A, CsrView T, const BlockRange* __restrict__ blocks) {
   const BlockRange br = blocks[blockIdx.x];
@@ -120,10 +114,10 @@ __global__ void __launch_bounds__(kBlock) face_kernel(StepArgs A, CsrView T, con
   const f3 p0 = ld3(S.x, S.y, S.z, a), p1 = ld3(S.x, S.y, S.z, b), p2 = ld3(S.x, S.y, S.z, c);
   const f3 sum = face_force<M>(p0, p1, p2, ld3(S.vx, S.vy, S.vz, a), ld3(S.vx, S.vy, S.vz, b),
                                ld3(S.vx, S.vy, S.vz, c), P);
-  T.face_sum[f] = make_float4(sum.x, sum.y, sum.z, 0.0f);
-  T.face_w[3 * f] = corner_weight<M>(p0, p1, p2);
-  T.face_w[3 * f + 1] = corner_weight<M>(p1, p2, p0);
-  T.face_w[3 * f + 2] = corner_weight<M>(p2, p0, p1);
+  const float w0 = corner_weight<M>(p0, p1, p2), w1 = corner_weight<M>(p1, p2, p0), w2 = corner_weight<M>(p2, p0, p1);
+  T.face_c[T.f_slot[3 * f]] = make_float4(w0 * sum.x, w0 * sum.y, w0 * sum.z, 0.0f);
+  T.face_c[T.f_slot[3 * f + 1]] = make_float4(w1 * sum.x, w1 * sum.y, w1 * sum.z, 0.0f);
+  T.face_c[T.f_slot[3 * f + 2]] = make_float4(w2 * sum.x, w2 * sum.y, w2 * sum.z, 0.0f);
 }
 
 // The four forces of each torsional hinge, once per hinge.

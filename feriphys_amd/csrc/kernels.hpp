@@ -31,11 +31,12 @@ struct HingeEnt {
 struct CsrView {
   const uint32_t* s_row;  // N+1: incident struts of each vertex, in strut-index order
   const StrutEnt* s_ent;
-  const uint32_t* f_row;  // N+1: incident faces of each vertex, in face-index order
-  const uint32_t* f_ent;  // (face << 2) | corner
-  const uint32_t* f_v;    // 3 global vertex ids per face
-  float4* face_sum;       // per face: drag + lift (x, y, z), written by the face kernel
-  float* face_w;          // per face corner: angle / PI, written by the face kernel
+  const uint32_t* f_row;   // N+1: incident faces of each vertex, in face-index order
+  const uint32_t* f_v;     // 3 global vertex ids per face
+  const uint32_t* f_slot;  // 3 per face: the position of (face, corner) in its vertex's f_row list
+  // per f_row entry: that corner's share of its face's drag + lift, (angle / PI) * (drag + lift),
+  // written by the face kernel at f_slot, so that a vertex reads its shares as one contiguous run
+  float4* face_c;
   // torsional hinges in the step (nullptr: none): per vertex its (hinge << 2 | role) entries in
   // strut order, and the 4 forces per hinge written by the hinge kernel
   const uint32_t* t_row;

@@ -8,7 +8,9 @@
 //   mass          f32 per vertex (read only for meshes whose masses are not uniform)
 //   pins          1 bit per vertex
 //   CSR           per vertex: incident struts (neighbour, rest length, k, c) in strut order and
-//                 incident faces ((face << 2) | corner) in face order; 3 vertex ids per face
+//                 one float4 per incident face (that corner's share of the face's drag + lift,
+//                 written by the face kernel) in face order; per face 3 vertex ids and the
+//                 positions of its corners in those per-vertex runs
 //   obstacles     faces (corners, unit normal, dropped axis) + a uniform grid over them
 #include <hip/hip_runtime.h>
 
@@ -83,7 +85,48 @@ struct MeshSlot {
   uint32_t blk_begin = 0, blk_end = 0;
   uint32_t fblk_begin = 0, fblk_end = 0;  // face blocks (range in fp_sim::h_face_blocks)
   uint32_t tile_begin = 0, tile_end = 0;  // range in fp_sim::mesh_tiles
+  // Device storage order of a general mesh's vertices (empty: the reference order, as for cloths):
+  // perm[d] = the reference index stored at device slot vbase + d, pos = its inverse.
+  std::vector<uint32_t> perm, pos;
+  uint64_t dev(uint64_t i) const { return vbase + (pos.empty() ? i : pos[i]); }
+  uint64_t ref(uint64_t d) const { return perm.empty() ? d : perm[d]; }
 };
+
+// Reverse Cuthill-McKee order of a mesh's strut graph: breadth-first from a lowest-degree vertex,
+// neighbours in ascending degree, each connected component in turn, the whole order reversed.
+// Neighbours then sit within a narrow band of indices, so the per-vertex gathers of one
+// workgroup read few cache lines, most of them shared with the neighbouring workgroups.
+std::vector<uint32_t> rcm_order(const fp::HostMesh& h) {
+  const uint64_t n = h.n;
+  std::vector<uint32_t> row(n + 1, 0), adj(2 * h.n_struts()), fill;
+  for (uint64_t k = 0; k < h.n_struts(); ++k) { row[h.s_v0[k] + 1]++; row[h.s_v1[k] + 1]++; }
+  for (uint64_t i = 0; i < n; ++i) row[i + 1] += row[i];
+  fill.assign(row.begin(), row.end() - 1);
+  for (uint64_t k = 0; k < h.n_struts(); ++k) { adj[fill[h.s_v0[k]]++] = h.s_v1[k]; adj[fill[h.s_v1[k]]++] = h.s_v0[k]; }
+  auto deg = [&](uint32_t v) { return row[v + 1] - row[v]; };
+  std::vector<uint32_t> by_deg(n), order;
+  for (uint64_t i = 0; i < n; ++i) by_deg[i] = (uint32_t)i;
+  std::stable_sort(by_deg.begin(), by_deg.end(), [&](uint32_t a, uint32_t b) { return deg(a) < deg(b); });
+  std::vector<uint8_t> seen(n, 0);
+  std::vector<uint32_t> nb;
+  order.reserve(n);
+  for (uint32_t start : by_deg) {
+    if (seen[start]) continue;
+    seen[start] = 1;
+    const size_t head = order.size();
+    order.push_back(start);
+    for (size_t q = head; q < order.size(); ++q) {
+      const uint32_t v = order[q];
+      nb.clear();
+      for (uint32_t e = row[v]; e < row[v + 1]; ++e)
+        if (!seen[adj[e]]) { seen[adj[e]] = 1; nb.push_back(adj[e]); }
+      std::stable_sort(nb.begin(), nb.end(), [&](uint32_t a, uint32_t b) { return deg(a) < deg(b); });
+      order.insert(order.end(), nb.begin(), nb.end());
+    }
+  }
+  std::reverse(order.begin(), order.end());
+  return order;
+}
 
 // Default tile shape for a cloth of this size: 16x16 (fastest measured on the 1024 x 1024 cloth,
 // one vertex per thread); 7x31 for short wide cloths.
@@ -104,10 +147,9 @@ struct fp_sim {
   bool forces_dirty = false;
   DBuf<float> mass;
   DBuf<uint32_t> pins;
-  DBuf<uint32_t> s_row, f_row, f_ent, f_v;
+  DBuf<uint32_t> s_row, f_row, f_v, f_slot;
   DBuf<fp::StrutEnt> s_ent;
-  DBuf<float4> face_sum;
-  DBuf<float> face_w;
+  DBuf<float4> face_c;
   // face blocks of the meshes on the CSR kernel (BlockRange over global face ids), all meshes'
   // face blocks (for the split API, indexed per mesh by fblk_begin/fblk_end)
   DBuf<fp::BlockRange> csr_face_blocks, all_face_blocks;
@@ -195,7 +237,7 @@ struct fp_sim {
 
   fp::CsrView csr(bool with_torsion = false) const {
     const bool t = (with_torsion || torsion_on) && n_hinges > 0;
-    return fp::CsrView{s_row.p, s_ent.p, f_row.p, f_ent.p, f_v.p, face_sum.p, face_w.p,
+    return fp::CsrView{s_row.p, s_ent.p, f_row.p, f_v.p, f_slot.p, face_c.p,
                        t ? t_row.p : nullptr, t ? t_ent.p : nullptr, hinge_f.p};
   }
 };
@@ -257,15 +299,16 @@ void cloth_tiles(uint32_t mesh, const fp::ClothDesc& d, int shape, std::vector<f
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), each XCD with its
-// own L2.  Launch order is permuted so that XCD k receives the k-th contiguous eighth of the
-// tiles: vertically adjacent tiles, which read each other's halo rows, then share an L2.
-std::vector<fp::TileRange> xcd_order(const std::vector<fp::TileRange>& t) {
-  const size_t n = t.size(), per = (n + 7) / 8;
-  std::vector<fp::TileRange> out;
-  out.reserve(n);
-  for (size_t b = 0; b < 8 * per; ++b) {
-    const size_t logical = (b % 8) * per + b / 8;
-    if (logical < n) out.push_back(t[logical]);
+// own L2.  Launch order is permuted so that XCD x receives the x-th contiguous eighth of the work
+// list (the first n % 8 eighths one item longer): vertically adjacent tiles, which read each
+// other's halo rows, and neighbouring vertex blocks of a CSR mesh then share an L2.
+template <class T>
+std::vector<T> xcd_order(const std::vector<T>& t) {
+  const size_t n = t.size(), q = n / 8, r = n % 8;
+  std::vector<T> out(n);
+  for (size_t b = 0; b < n; ++b) {
+    const size_t x = b % 8, k = b / 8;
+    out[b] = t[x * q + (x < r ? x : r) + k];  // k < q + (x < r) for every b < n
   }
   return out;
 }
@@ -299,6 +342,10 @@ int build_block_lists(fp_sim* s) {
   }
   // the surrogate's generic path (face kernel + per-vertex update) covers every mesh that is not
   // on the tiled kernel
+  c = xcd_order(c);
+  cf = xcd_order(cf);
+  sb = xcd_order(sb);
+  sf = xcd_order(sf);
   s->n_sur_blocks = (uint32_t)sb.size();
   s->n_sur_face_blocks = (uint32_t)sf.size();
   HIP_TRY(s->sur_blocks.upload(sb.data(), sb.size()));
@@ -713,6 +760,11 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     Fent += 3 * slot.host.n_faces();
     refresh_mass(slot, slot.host.rec.data(), slot.host.n);
     if (slot.host.cloth.valid) slot.tile_shape = default_tile_shape(slot.host.cloth.rows, slot.host.cloth.cols);
+    else if (slot.host.n > 1) {
+      slot.perm = rcm_order(slot.host);
+      slot.pos.assign(slot.host.n, 0);
+      for (uint64_t d = 0; d < slot.host.n; ++d) slot.pos[slot.perm[d]] = (uint32_t)d;
+    }
     s->meshes.push_back(std::move(slot));
   }
   if (N >= (1ull << 32) || F >= (1ull << 30) || Sdir >= (1ull << 32) || Fent >= (1ull << 32))
@@ -725,16 +777,16 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     std::vector<float> comp(N), m(N);
     for (int k = 0; k < 6; ++k) {
       for (auto& ms_ : s->meshes)
-        for (uint64_t i = 0; i < ms_.host.n; ++i) comp[ms_.vbase + i] = ms_.host.rec[10 * i + 1 + k];
+        for (uint64_t i = 0; i < ms_.host.n; ++i) comp[ms_.dev(i)] = ms_.host.rec[10 * i + 1 + k];
       HIP_TRY(s->state[0][k].upload(comp.data(), N));
       HIP_TRY(s->state[1][k].alloc(N));
     }
     for (auto& ms_ : s->meshes)
-      for (uint64_t i = 0; i < ms_.host.n; ++i) m[ms_.vbase + i] = ms_.host.rec[10 * i];
+      for (uint64_t i = 0; i < ms_.host.n; ++i) m[ms_.dev(i)] = ms_.host.rec[10 * i];
     HIP_TRY(s->mass.upload(m.data(), N));
     for (int k = 0; k < 3; ++k) {
       for (auto& ms_ : s->meshes)
-        for (uint64_t i = 0; i < ms_.host.n; ++i) comp[ms_.vbase + i] = ms_.host.rec[10 * i + 7 + k];
+        for (uint64_t i = 0; i < ms_.host.n; ++i) comp[ms_.dev(i)] = ms_.host.rec[10 * i + 7 + k];
       HIP_TRY(s->force[k].upload(comp.data(), N));
     }
     bool nonzero_f = false;
@@ -747,7 +799,7 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     std::vector<uint32_t> bits((N + 31) / 32 + 1, 0u);
     for (auto& ms_ : s->meshes)
       for (uint64_t p : ms_.host.pins) {
-        uint64_t g = ms_.vbase + p;
+        uint64_t g = ms_.dev(p);
         bits[g >> 5] |= 1u << (g & 31);
       }
     HIP_TRY(s->pins.upload(bits.data(), bits.size()));
@@ -759,37 +811,48 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     for (auto& ms_ : s->meshes) {
       const auto& h = ms_.host;
       for (uint64_t k = 0; k < h.n_struts(); ++k) {
-        srow[ms_.vbase + h.s_v0[k] + 1]++;
-        srow[ms_.vbase + h.s_v1[k] + 1]++;
+        srow[ms_.dev(h.s_v0[k]) + 1]++;
+        srow[ms_.dev(h.s_v1[k]) + 1]++;
       }
-      for (uint64_t k = 0; k < h.f_v.size(); ++k) {
-        frow[ms_.vbase + h.f_v[k] + 1]++;
-        fv[3 * ms_.fbase + k] = (uint32_t)(ms_.vbase + h.f_v[k]);
-      }
+      for (uint64_t k = 0; k < h.f_v.size(); ++k) frow[ms_.dev(h.f_v[k]) + 1]++;
     }
     for (uint64_t i = 0; i < N; ++i) { srow[i + 1] += srow[i]; frow[i + 1] += frow[i]; }
     std::vector<fp::StrutEnt> sent(srow[N]);
-    std::vector<uint32_t> fent(frow[N]);
+    std::vector<uint32_t> fslot(3 * F);
     std::vector<uint32_t> sfill(srow.begin(), srow.end() - 1), ffill(frow.begin(), frow.end() - 1);
     for (auto& ms_ : s->meshes) {
       const auto& h = ms_.host;
       for (uint64_t k = 0; k < h.n_struts(); ++k) {
-        uint32_t a = (uint32_t)(ms_.vbase + h.s_v0[k]), b = (uint32_t)(ms_.vbase + h.s_v1[k]);
+        uint32_t a = (uint32_t)ms_.dev(h.s_v0[k]), b = (uint32_t)ms_.dev(h.s_v1[k]);
         sent[sfill[a]++] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
         sent[sfill[b]++] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
       }
-      for (uint64_t t = 0; t < h.n_faces(); ++t)
+      // faces are stored (for the face kernel) in ascending order of their lowest device vertex id,
+      // so that neighbouring threads gather neighbouring vertices and write neighbouring runs; the
+      // runs themselves list each vertex's faces in face-index order
+      std::vector<uint32_t> slot3(3 * h.n_faces()), lo(h.n_faces()), at(h.n_faces());
+      for (uint64_t t = 0; t < h.n_faces(); ++t) {
+        uint32_t m = 0xffffffffu;
         for (uint32_t j = 0; j < 3; ++j) {
-          uint32_t v = (uint32_t)(ms_.vbase + h.f_v[3 * t + j]);
-          fent[ffill[v]++] = (uint32_t)(((ms_.fbase + t) << 2) | j);
+          uint32_t v = (uint32_t)ms_.dev(h.f_v[3 * t + j]);
+          slot3[3 * t + j] = ffill[v]++;
+          m = std::min(m, v);
+        }
+        lo[t] = m;
+        at[t] = (uint32_t)t;
+      }
+      std::stable_sort(at.begin(), at.end(), [&](uint32_t a, uint32_t b) { return lo[a] < lo[b]; });
+      for (uint64_t q = 0; q < h.n_faces(); ++q)
+        for (uint32_t j = 0; j < 3; ++j) {
+          fv[3 * (ms_.fbase + q) + j] = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)at[q] + j]);
+          fslot[3 * (ms_.fbase + q) + j] = slot3[3 * (uint64_t)at[q] + j];
         }
     }
     HIP_TRY(s->s_row.upload(srow.data(), srow.size()));
     HIP_TRY(s->s_ent.upload(sent.data(), sent.size()));
-    HIP_TRY(s->face_sum.alloc(F));
-    HIP_TRY(s->face_w.alloc(3 * F));
+    HIP_TRY(s->face_c.alloc(frow[N]));
     HIP_TRY(s->f_row.upload(frow.data(), frow.size()));
-    HIP_TRY(s->f_ent.upload(fent.data(), fent.size()));
+    HIP_TRY(s->f_slot.upload(fslot.data(), fslot.size()));
     HIP_TRY(s->f_v.upload(fv.data(), fv.size()));
   }
 
@@ -810,10 +873,10 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
           return a;
         };
         fp::HingeEnt e{};
-        e.v[0] = (uint32_t)(ms_.vbase + a);
-        e.v[1] = (uint32_t)(ms_.vbase + b);
-        e.v[2] = (uint32_t)(ms_.vbase + other(h.s_f0[k]));
-        e.v[3] = (uint32_t)(ms_.vbase + other(h.s_f1[k]));
+        e.v[0] = (uint32_t)ms_.dev(a);
+        e.v[1] = (uint32_t)ms_.dev(b);
+        e.v[2] = (uint32_t)ms_.dev(other(h.s_f0[k]));
+        e.v[3] = (uint32_t)ms_.dev(other(h.s_f1[k]));
         e.k = h.t_k[k];
         e.c = h.t_c[k];
         e.rest = h.t_rest[k];
@@ -1043,7 +1106,7 @@ int fp_sim_get_points(fp_sim* s, uint64_t mesh, float* rec) {
   std::vector<float> comp(n);
   auto get = [&](const float* d, int slot) -> int {
     HIP_TRY(hipMemcpy(comp.data(), d + b, n * sizeof(float), hipMemcpyDeviceToHost));
-    for (uint64_t i = 0; i < n; ++i) rec[10 * i + slot] = comp[i];
+    for (uint64_t i = 0; i < n; ++i) rec[10 * i + slot] = comp[m.dev(i) - b];
     return FP_OK;
   };
   if (int e = get(s->mass.p, 0)) return e;
@@ -1063,7 +1126,7 @@ int fp_sim_set_points(fp_sim* s, uint64_t mesh, const float* rec, uint64_t n) {
   std::vector<float> comp(n);
   bool nonzero_f = false;
   auto put = [&](float* d, int slot) -> int {
-    for (uint64_t i = 0; i < n; ++i) comp[i] = rec[10 * i + slot];
+    for (uint64_t i = 0; i < n; ++i) comp[m.dev(i) - m.vbase] = rec[10 * i + slot];
     HIP_TRY(hipMemcpy(d + m.vbase, comp.data(), n * sizeof(float), hipMemcpyHostToDevice));
     return FP_OK;
   };
@@ -1143,7 +1206,13 @@ int fp_sim_update_points(fp_sim* s, uint64_t mesh, const float* rec, uint64_t n)
   if (n && !rec) return fail(FP_EINVAL, "records is NULL");
   if (n == 0) return FP_OK;
   HIP_TRY(hipStreamSynchronize(s->stream));
-  HIP_TRY(s->tmp_rec.upload(rec, 10 * n));
+  if (m.perm.empty()) {
+    HIP_TRY(s->tmp_rec.upload(rec, 10 * n));
+  } else {  // records in device order
+    std::vector<float> r(10 * n);
+    for (uint64_t d = 0; d < n; ++d) std::memcpy(&r[10 * d], rec + 10 * (uint64_t)m.perm[d], 10 * sizeof(float));
+    HIP_TRY(s->tmp_rec.upload(r.data(), r.size()));
+  }
   fp::StepArgs a = s->args(true);
   HIP_TRY(fp::launch_update_points(a, s->tmp_rec.p, (uint32_t)m.vbase, s->mass.p, s->all_blocks.p + m.blk_begin,
                                    m.blk_end - m.blk_begin, s->stream));
@@ -1152,7 +1221,7 @@ int fp_sim_update_points(fp_sim* s, uint64_t mesh, const float* rec, uint64_t n)
   // masses may have changed: re-derive the uniform-mass flag from the device copy
   std::vector<float> mm(n), r10(10 * n, 0.0f);
   HIP_TRY(hipMemcpy(mm.data(), s->mass.p + m.vbase, n * sizeof(float), hipMemcpyDeviceToHost));
-  for (uint64_t i = 0; i < n; ++i) r10[10 * i] = mm[i];
+  for (uint64_t i = 0; i < n; ++i) r10[10 * i] = mm[i];  // device order: only the uniformity matters
   refresh_mass(m, r10.data(), n);
   if (int e = upload_params(s)) return e;
   return build_block_lists(s);
@@ -1226,7 +1295,7 @@ int build_slots(fp_sim* s) {
     for (uint64_t p : h.pins) pin[p] = 1;
     for (uint64_t i = 0; i < h.n; ++i) {
       mesh_of[ms.vbase + i] = mi;
-      if (!pin[i]) ids.push_back((uint32_t)(ms.vbase + i));
+      if (!pin[ms.ref(i)]) ids.push_back((uint32_t)(ms.vbase + i));
     }
     const bool grid = h.cloth.valid;
     const int64_t C = h.cloth.cols;
@@ -1234,7 +1303,7 @@ int build_slots(fp_sim* s) {
       if (!(h.s_len[k] > 0.0f)) continue;
       for (int side = 0; side < 2; ++side) {
         const uint32_t self = side ? h.s_v1[k] : h.s_v0[k], other = side ? h.s_v0[k] : h.s_v1[k];
-        const uint64_t gi = ms.vbase + self;
+        const uint64_t gi = ms.dev(self);
         int slot = -1;
         if (grid) {
           const int64_t dr = (int64_t)(other / C) - (int64_t)(self / C), dc = (int64_t)(other % C) - (int64_t)(self % C);
@@ -1244,7 +1313,7 @@ int build_slots(fp_sim* s) {
           slot = used[gi]++;
         }
         if (slot < 0) continue;
-        S.h_slots[gi * fp::kSlots + slot] = fp::SlotEnt{(uint32_t)(ms.vbase + other), h.s_len[k]};
+        S.h_slots[gi * fp::kSlots + slot] = fp::SlotEnt{(uint32_t)ms.dev(other), h.s_len[k]};
         sk[gi * fp::kSlots + slot] = h.s_k[k];
       }
     }
@@ -1496,7 +1565,7 @@ int fp_sim_collect_samples(fp_sim* s, fp_surrogate* n, uint64_t n_big, float noi
       std::vector<uint8_t> pin(ms.host.n, 0);
       for (uint64_t q : ms.host.pins) pin[q] = 1;
       for (uint64_t i = 0; i < ms.host.n; ++i)
-        if (!pin[i]) ids.push_back((uint32_t)(ms.vbase + i));
+        if (!pin[ms.ref(i)]) ids.push_back((uint32_t)(ms.vbase + i));
     }
     n_ids = ids.size();
     HIP_TRY(ids_now.upload(ids.data(), ids.size()));
@@ -1545,7 +1614,7 @@ int fp_sim_surrogate_features(fp_sim* s, uint64_t mesh, float* features) {
   if (int e = prepare_surrogate(s, s->sur.cfg)) return e;
   const MeshSlot& m = s->meshes[mesh];
   std::vector<uint32_t> ids(m.host.n);
-  for (uint64_t i = 0; i < m.host.n; ++i) ids[i] = (uint32_t)(m.vbase + i);
+  for (uint64_t i = 0; i < m.host.n; ++i) ids[i] = (uint32_t)m.dev(i);  // rows in reference order
   DBuf<uint32_t> d_ids;
   DBuf<float> out;
   HIP_TRY(d_ids.upload(ids.data(), ids.size()));
@@ -1569,7 +1638,10 @@ int fp_sim_surrogate_accel(fp_sim* s, uint64_t mesh, float* accel) {
                              S.grid, S.variant, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
   const MeshSlot& m = s->meshes[mesh];
-  HIP_TRY(hipMemcpy(accel, S.da.p + 3 * m.vbase, 3 * m.host.n * sizeof(float), hipMemcpyDeviceToHost));
+  std::vector<float> da(3 * m.host.n);
+  HIP_TRY(hipMemcpy(da.data(), S.da.p + 3 * m.vbase, da.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (uint64_t i = 0; i < m.host.n; ++i)
+    for (int c = 0; c < 3; ++c) accel[3 * i + c] = da[3 * (m.dev(i) - m.vbase) + c];
   return FP_OK;
 }
 
@@ -1581,8 +1653,8 @@ int fp_sim_surrogate_slots(fp_sim* s, uint64_t mesh, int64_t* nbr, float* rest) 
   const MeshSlot& m = s->meshes[mesh];
   for (uint64_t i = 0; i < m.host.n; ++i)
     for (int j = 0; j < fp::kSlots; ++j) {
-      const fp::SlotEnt e = s->sur.h_slots[(m.vbase + i) * fp::kSlots + j];
-      nbr[i * fp::kSlots + j] = e.nbr == fp::kNoSlot ? -1 : (int64_t)(e.nbr - m.vbase);
+      const fp::SlotEnt e = s->sur.h_slots[m.dev(i) * fp::kSlots + j];
+      nbr[i * fp::kSlots + j] = e.nbr == fp::kNoSlot ? -1 : (int64_t)m.ref(e.nbr - m.vbase);
       rest[i * fp::kSlots + j] = e.nbr == fp::kNoSlot ? 0.0f : e.rest;
     }
   return FP_OK;

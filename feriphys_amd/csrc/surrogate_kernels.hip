@@ -219,10 +219,8 @@ __global__ void __launch_bounds__(kBlock) sur_integrate_kernel(StepArgs A, CsrVi
   const float m = P.uniform_mass ? P.mass : A.mass[i];
   f3 Ff = f3{0.0f, 0.0f, 0.0f};
   for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {
-    const uint32_t ent = T.f_ent[e];
-    const float4 sum = T.face_sum[ent >> 2];
-    const float w = T.face_w[3 * (ent >> 2) + (ent & 3u)];
-    Ff = add(Ff, f3{w * sum.x, w * sum.y, w * sum.z});
+    const float4 fc = T.face_c[e];
+    Ff = add(Ff, f3{fc.x, fc.y, fc.z});
   }
   f3 Fs = f3{0.0f, 0.0f, 0.0f};
   if (STRUTS)
@@ -387,10 +385,8 @@ __global__ void __launch_bounds__(kBlock) sur_samples_kernel(StepArgs A, CsrView
   const float m = P.uniform_mass ? P.mass : A.mass[i];
   f3 F = P.gravity;
   for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {
-    const uint32_t ent = T.f_ent[e];
-    const float4 sum = T.face_sum[ent >> 2];
-    const float w = T.face_w[3 * (ent >> 2) + (ent & 3u)];
-    F = add(F, f3{w * sum.x, w * sum.y, w * sum.z});
+    const float4 fc = T.face_c[e];
+    F = add(F, f3{fc.x, fc.y, fc.z});
   }
   if (STRUTS)
     for (uint32_t e = T.s_row[i], ee = T.s_row[i + 1]; e < ee; ++e) {

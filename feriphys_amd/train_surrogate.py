@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--groups", type=int, default=4, help="trajectories are simulated in this many batches")
     ap.add_argument("--large", type=int, default=16,
                     help="extra cloths with sides in [192, 512] (the tension of long hanging cloths)")
+    ap.add_argument("--sphere-groups", type=int, default=0,
+                    help="of the cloth groups, how many fall onto a sphere collider (contact data)")
     ap.add_argument("--scene", choices=["cloths", "surfaces"], default="cloths",
                     help="cloths: grid cloths (the 1024 x 1024 cloth's network); surfaces: icospheres (config4's)")
     ap.add_argument("--meshes", type=int, default=400, help="icospheres per training scene (--scene surfaces)")
@@ -85,7 +87,8 @@ def main():
     per = (args.cloths + args.groups - 1) // args.groups
     for g in range(args.groups if args.scene == "cloths" else 0):
         n = min(per, args.cloths - g * per)
-        sc = wl.surrogate_training_scene(n_cloths=n, seed=1000 * (args.seed + 1) + g, smin=args.smin, smax=args.smax)
+        sc = wl.surrogate_training_scene(n_cloths=n, seed=1000 * (args.seed + 1) + g, smin=args.smin, smax=args.smax,
+                                         sphere=g < args.sphere_groups)
         sim = wl.build_gpu(sc)
         net.collect(sim, n_big=args.big, noise=args.noise, seed=args.seed * 7919 + g)
         del sim
@@ -99,7 +102,8 @@ def main():
             print(f"group {g}: {args.meshes} icospheres, {net.sample_count()} samples so far "
                   f"({time.time() - t0:.1f} s)", flush=True)
     if args.large and args.scene == "cloths":
-        sc = wl.surrogate_training_scene(n_cloths=args.large, seed=7777 + args.seed, smin=192, smax=512)
+        sc = wl.surrogate_training_scene(n_cloths=args.large, seed=7777 + args.seed, smin=192, smax=512,
+                                         sphere=args.sphere_groups > 0)
         sim = wl.build_gpu(sc)
         net.collect(sim, n_big=args.big, noise=args.noise, seed=args.seed * 7919 + 99)
         del sim

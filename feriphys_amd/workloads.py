@@ -239,12 +239,15 @@ def springy_cube():
     return pos, idx, 80.0, 999999.0, 700.0, (7000.0, 7000.0), extra, floor, [0, 1, 2, 0, 2, 3]
 
 
-def surrogate_training_scene(n_cloths: int = 128, seed: int = 0, smin: int = 12, smax: int = 48) -> Scene:
+def surrogate_training_scene(n_cloths: int = 128, seed: int = 0, smin: int = 12, smax: int = 48,
+                             sphere: bool = False) -> Scene:
     """Cloths whose plain trajectories train the surrogate: rows and cols each log-uniform in
     [smin, smax], one full edge pinned (chosen at random), config-0 stiffness, mass, spacing,
     gravity, drag and lift; per-cloth wind with direction uniform on the upper hemisphere and
     magnitude uniform in [0, 3] m/s; RK4 at dt = 1/600 s; +-1e-3 m jitter.  Corner pins are left
-    out: a cloth hanging from corners diverges in the reference method itself (see config0)."""
+    out: a cloth hanging from corners diverges in the reference method itself (see config0).
+    sphere: one icosphere collider (radius 0.3-2 m, apex 0.1-0.6 m below the cloths, which all
+    start centred on the origin) that every cloth falls onto."""
     rng = np.random.default_rng(seed)
     lo, hi = math.log(smin), math.log(smax + 1)
     cloths, winds = [], []
@@ -260,7 +263,12 @@ def surrogate_training_scene(n_cloths: int = 128, seed: int = 0, smin: int = 12,
         d[1] = abs(d[1])
         winds.append(tuple(float(F32(x)) for x in d * rng.uniform(0.0, 3.0)))
     base = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.01), (0.0, 0.0, 0.0), 0.2, 0.5, 0.95, 0.3).f32()
-    sc = Scene(cloths, base, [], 1e-3, seed, 600, f"{n_cloths} training cloths")
+    obstacles = []
+    if sphere:
+        r = float(rng.uniform(0.3, 2.0))
+        obstacles = [sphere_obstacle(r, (0.0, -(float(rng.uniform(0.1, 0.6)) + r), 0.0), 2)]
+        base = ConfigSpec(1, 1.0 / 600.0, gravity_force(0.01), (0.0, 0.0, 0.0), 0.2, 0.5, 0.3, 0.2).f32()
+    sc = Scene(cloths, base, obstacles, 1e-3, seed, 600, f"{n_cloths} training cloths")
     sc.mesh_winds = winds
     return sc
 
