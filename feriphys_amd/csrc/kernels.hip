@@ -48,12 +48,13 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
   // then neighbour states) are all issued before the first force is added, so one row costs a few
   // memory latencies rather than two per strut; the additions stay in strut order.
   {
-    const uint32_t e0 = T.s_row[i], ee = T.s_row[i + 1];
-    for (uint32_t e = e0; e < ee; e += 4) {
+    const StrutEnt* ent = T.s_ent + T.s_first[i];
+    const uint32_t ns = T.s_cnt[i];
+    for (uint32_t e = 0; e < ns; e += 4) {
       StrutEnt se[4];
       f3 q[4], w[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) se[j] = T.s_ent[e + j < ee ? e + j : e];
+      for (int j = 0; j < 4; ++j) se[j] = ent[kSlice * (e + j < ns ? e + j : e)];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         q[j] = ld3(S.x, S.y, S.z, se[j].nbr);
@@ -61,7 +62,7 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (e + j < ee) strut_accumulate<M>(F, p, v, q[j], w[j], se[j].k, se[j].c, se[j].len);
+        if (e + j < ns) strut_accumulate<M>(F, p, v, q[j], w[j], se[j].k, se[j].c, se[j].len);
     }
   }
   // apply_torsional_forces (:553-627), when the step includes it (the reference comments the call
@@ -79,16 +80,17 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
     }
   }
   // apply_face_forces (:629-654): the face kernel has stored each corner's weight * (drag + lift)
-  // in this vertex's run of f_row entries; they are added in face order
+  // in this vertex's face entries; they are added in face order
   {
-    const uint32_t e0 = T.f_row[i], ee = T.f_row[i + 1];
-    for (uint32_t e = e0; e < ee; e += 4) {
+    const float4* fe = T.face_c + T.f_first[i];
+    const uint32_t nf = T.f_cnt[i];
+    for (uint32_t e = 0; e < nf; e += 4) {
       float4 fc[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) fc[j] = T.face_c[e + j < ee ? e + j : e];
+      for (int j = 0; j < 4; ++j) fc[j] = fe[kSlice * (e + j < nf ? e + j : e)];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (e + j < ee) F = add(F, f3{fc[j].x, fc[j].y, fc[j].z});
+        if (e + j < nf) F = add(F, f3{fc[j].x, fc[j].y, fc[j].z});
     }
   }
   const bool pin = pinned(A.pins, i);
@@ -102,7 +104,7 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
 
 // Drag + lift (springy_mesh.rs:631-646) and the three corner weights angle / PI (:647-649) of every
 // face of the CSR meshes, once per face; corner k's share weight_k * (drag + lift) (the product of
-// :650-652) goes to its vertex's run of f_row entries.
+// :650-652) goes to its vertex's face entry.
 template <class M>
 __global__ void __launch_bounds__(kBlock) face_kernel(StepArgs A, CsrView T, const BlockRange* __restrict__ blocks) {
   const BlockRange br = blocks[blockIdx.x];

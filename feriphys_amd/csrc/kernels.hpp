@@ -27,18 +27,24 @@ struct HingeEnt {
   float k, c, rest, pad;
 };
 
-// Vertex-centric CSR of a packed set of meshes (global vertex and face ids).
+// Vertex-centric incidence lists of a packed set of meshes (global device vertex and face ids), in
+// a sliced layout: the vertices of one mesh are cut into slices of kSlice consecutive ids (one
+// wavefront), and entry j of a vertex is at first[v] + kSlice * j, so that one load instruction of
+// a wavefront reads kSlice consecutive entries.
+constexpr uint32_t kSlice = 64;
 struct CsrView {
-  const uint32_t* s_row;  // N+1: incident struts of each vertex, in strut-index order
+  const uint32_t* s_first;  // per vertex: its strut entries, in strut-index order
+  const uint32_t* s_cnt;
   const StrutEnt* s_ent;
-  const uint32_t* f_row;   // N+1: incident faces of each vertex, in face-index order
-  const uint32_t* f_v;     // 3 global vertex ids per face
-  const uint32_t* f_slot;  // 3 per face: the position of (face, corner) in its vertex's f_row list
-  // per f_row entry: that corner's share of its face's drag + lift, (angle / PI) * (drag + lift),
-  // written by the face kernel at f_slot, so that a vertex reads its shares as one contiguous run
+  const uint32_t* f_first;  // per vertex: its incident faces, in face-index order
+  const uint32_t* f_cnt;
+  const uint32_t* f_v;      // 3 global vertex ids per face
+  const uint32_t* f_slot;   // 3 per face: the position of (face, corner) among its vertex's entries
+  // per face entry: that corner's share of its face's drag + lift, (angle / PI) * (drag + lift),
+  // written by the face kernel at f_slot
   float4* face_c;
   // torsional hinges in the step (nullptr: none): per vertex its (hinge << 2 | role) entries in
-  // strut order, and the 4 forces per hinge written by the hinge kernel
+  // strut order (plain CSR, N+1 row pointers), and the 4 forces per hinge written by the hinge kernel
   const uint32_t* t_row;
   const uint32_t* t_ent;
   float4* hinge_f;

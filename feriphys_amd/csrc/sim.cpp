@@ -7,10 +7,11 @@
 //                 update_points) or when a step starts from non-zero forces
 //   mass          f32 per vertex (read only for meshes whose masses are not uniform)
 //   pins          1 bit per vertex
-//   CSR           per vertex: incident struts (neighbour, rest length, k, c) in strut order and
+//   incidence     per vertex: incident struts (neighbour, rest length, k, c) in strut order and
 //                 one float4 per incident face (that corner's share of the face's drag + lift,
-//                 written by the face kernel) in face order; per face 3 vertex ids and the
-//                 positions of its corners in those per-vertex runs
+//                 written by the face kernel) in face order, sliced 64 vertices at a time (entry j
+//                 of a vertex at first + 64 j); per face 3 vertex ids and the positions of its
+//                 corners' entries
 //   obstacles     faces (corners, unit normal, dropped axis) + a uniform grid over them
 #include <hip/hip_runtime.h>
 
@@ -147,7 +148,7 @@ struct fp_sim {
   bool forces_dirty = false;
   DBuf<float> mass;
   DBuf<uint32_t> pins;
-  DBuf<uint32_t> s_row, f_row, f_v, f_slot;
+  DBuf<uint32_t> s_first, s_cnt, f_first, f_cnt, f_v, f_slot;
   DBuf<fp::StrutEnt> s_ent;
   DBuf<float4> face_c;
   // face blocks of the meshes on the CSR kernel (BlockRange over global face ids), all meshes'
@@ -237,7 +238,7 @@ struct fp_sim {
 
   fp::CsrView csr(bool with_torsion = false) const {
     const bool t = (with_torsion || torsion_on) && n_hinges > 0;
-    return fp::CsrView{s_row.p, s_ent.p, f_row.p, f_v.p, f_slot.p, face_c.p,
+    return fp::CsrView{s_first.p, s_cnt.p, s_ent.p, f_first.p, f_cnt.p, f_v.p, f_slot.p, face_c.p,
                        t ? t_row.p : nullptr, t ? t_ent.p : nullptr, hinge_f.p};
   }
 };
@@ -805,37 +806,52 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     HIP_TRY(s->pins.upload(bits.data(), bits.size()));
   }
 
-  // CSR (vertex-centric, incident struts in strut order, incident faces in face order)
+  // incidence lists (vertex-centric, incident struts in strut order, incident faces in face
+  // order) in the sliced layout of fp::CsrView
   {
-    std::vector<uint32_t> srow(N + 1, 0), frow(N + 1, 0), fv(3 * F);
+    std::vector<uint32_t> scnt(N, 0), fcnt(N, 0), sfirst(N), ffirst(N), fv(3 * F);
     for (auto& ms_ : s->meshes) {
       const auto& h = ms_.host;
       for (uint64_t k = 0; k < h.n_struts(); ++k) {
-        srow[ms_.dev(h.s_v0[k]) + 1]++;
-        srow[ms_.dev(h.s_v1[k]) + 1]++;
+        scnt[ms_.dev(h.s_v0[k])]++;
+        scnt[ms_.dev(h.s_v1[k])]++;
       }
-      for (uint64_t k = 0; k < h.f_v.size(); ++k) frow[ms_.dev(h.f_v[k]) + 1]++;
+      for (uint64_t k = 0; k < h.f_v.size(); ++k) fcnt[ms_.dev(h.f_v[k])]++;
     }
-    for (uint64_t i = 0; i < N; ++i) { srow[i + 1] += srow[i]; frow[i + 1] += frow[i]; }
-    std::vector<fp::StrutEnt> sent(srow[N]);
-    std::vector<uint32_t> fslot(3 * F);
-    std::vector<uint32_t> sfill(srow.begin(), srow.end() - 1), ffill(frow.begin(), frow.end() - 1);
+    // slice g of a mesh (device ids vbase + [kSlice g, kSlice (g+1))) takes kSlice x (its largest
+    // count) entries; lane l's entry j is at the slice base + kSlice j + l
+    auto slices = [&](const std::vector<uint32_t>& cnt, std::vector<uint32_t>& first) -> uint64_t {
+      uint64_t total = 0;
+      for (auto& ms_ : s->meshes)
+        for (uint64_t g0 = 0; g0 < ms_.host.n; g0 += fp::kSlice) {
+          const uint64_t g1 = std::min<uint64_t>(ms_.host.n, g0 + fp::kSlice);
+          uint32_t mx = 0;
+          for (uint64_t d = g0; d < g1; ++d) mx = std::max(mx, cnt[ms_.vbase + d]);
+          for (uint64_t d = g0; d < g1; ++d) first[ms_.vbase + d] = (uint32_t)(total + (d - g0));
+          total += (uint64_t)fp::kSlice * mx;
+        }
+      return total;
+    };
+    const uint64_t n_sent = slices(scnt, sfirst), n_fent = slices(fcnt, ffirst);
+    if (n_sent >= (1ull << 32) || n_fent >= (1ull << 32)) return fail(FP_EINVAL, "simulation too large for 32-bit ids");
+    std::vector<fp::StrutEnt> sent(n_sent + 1, fp::StrutEnt{0, 1.0f, 0.0f, 0.0f});
+    std::vector<uint32_t> fslot(3 * F), sfill(N, 0), ffill(N, 0);
     for (auto& ms_ : s->meshes) {
       const auto& h = ms_.host;
       for (uint64_t k = 0; k < h.n_struts(); ++k) {
         uint32_t a = (uint32_t)ms_.dev(h.s_v0[k]), b = (uint32_t)ms_.dev(h.s_v1[k]);
-        sent[sfill[a]++] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
-        sent[sfill[b]++] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
+        sent[sfirst[a] + fp::kSlice * sfill[a]++] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
+        sent[sfirst[b] + fp::kSlice * sfill[b]++] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
       }
       // faces are stored (for the face kernel) in ascending order of their lowest device vertex id,
-      // so that neighbouring threads gather neighbouring vertices and write neighbouring runs; the
-      // runs themselves list each vertex's faces in face-index order
+      // so that neighbouring threads gather neighbouring vertices and write neighbouring entries;
+      // each vertex's entries list its faces in face-index order
       std::vector<uint32_t> slot3(3 * h.n_faces()), lo(h.n_faces()), at(h.n_faces());
       for (uint64_t t = 0; t < h.n_faces(); ++t) {
         uint32_t m = 0xffffffffu;
         for (uint32_t j = 0; j < 3; ++j) {
           uint32_t v = (uint32_t)ms_.dev(h.f_v[3 * t + j]);
-          slot3[3 * t + j] = ffill[v]++;
+          slot3[3 * t + j] = ffirst[v] + fp::kSlice * ffill[v]++;
           m = std::min(m, v);
         }
         lo[t] = m;
@@ -848,10 +864,12 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
           fslot[3 * (ms_.fbase + q) + j] = slot3[3 * (uint64_t)at[q] + j];
         }
     }
-    HIP_TRY(s->s_row.upload(srow.data(), srow.size()));
+    HIP_TRY(s->s_first.upload(sfirst.data(), sfirst.size()));
+    HIP_TRY(s->s_cnt.upload(scnt.data(), scnt.size()));
     HIP_TRY(s->s_ent.upload(sent.data(), sent.size()));
-    HIP_TRY(s->face_c.alloc(frow[N]));
-    HIP_TRY(s->f_row.upload(frow.data(), frow.size()));
+    HIP_TRY(s->face_c.alloc(n_fent + 1));
+    HIP_TRY(s->f_first.upload(ffirst.data(), ffirst.size()));
+    HIP_TRY(s->f_cnt.upload(fcnt.data(), fcnt.size()));
     HIP_TRY(s->f_slot.upload(fslot.data(), fslot.size()));
     HIP_TRY(s->f_v.upload(fv.data(), fv.size()));
   }

@@ -218,14 +218,14 @@ __global__ void __launch_bounds__(kBlock) sur_integrate_kernel(StepArgs A, CsrVi
   }
   const float m = P.uniform_mass ? P.mass : A.mass[i];
   f3 Ff = f3{0.0f, 0.0f, 0.0f};
-  for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {
-    const float4 fc = T.face_c[e];
+  for (uint32_t e = 0, nf = T.f_cnt[i]; e < nf; ++e) {
+    const float4 fc = T.face_c[T.f_first[i] + kSlice * e];
     Ff = add(Ff, f3{fc.x, fc.y, fc.z});
   }
   f3 Fs = f3{0.0f, 0.0f, 0.0f};
   if (STRUTS)
-    for (uint32_t e = T.s_row[i], ee = T.s_row[i + 1]; e < ee; ++e) {
-      const StrutEnt se = T.s_ent[e];
+    for (uint32_t e = 0, ns = T.s_cnt[i]; e < ns; ++e) {
+      const StrutEnt se = T.s_ent[T.s_first[i] + kSlice * e];
       strut_accumulate<FastMath>(Fs, p, v, ld3(S.x, S.y, S.z, se.nbr), ld3(S.vx, S.vy, S.vz, se.nbr), se.k, se.c,
                                  se.len);
     }
@@ -384,13 +384,13 @@ __global__ void __launch_bounds__(kBlock) sur_samples_kernel(StepArgs A, CsrView
   for (int c = 0; c < kFeat; ++c) ok = ok && __builtin_isfinite(f[c]);
   const float m = P.uniform_mass ? P.mass : A.mass[i];
   f3 F = P.gravity;
-  for (uint32_t e = T.f_row[i], ee = T.f_row[i + 1]; e < ee; ++e) {
-    const float4 fc = T.face_c[e];
+  for (uint32_t e = 0, nf = T.f_cnt[i]; e < nf; ++e) {
+    const float4 fc = T.face_c[T.f_first[i] + kSlice * e];
     F = add(F, f3{fc.x, fc.y, fc.z});
   }
   if (STRUTS)
-    for (uint32_t e = T.s_row[i], ee = T.s_row[i + 1]; e < ee; ++e) {
-      const StrutEnt se = T.s_ent[e];
+    for (uint32_t e = 0, ns = T.s_cnt[i]; e < ns; ++e) {
+      const StrutEnt se = T.s_ent[T.s_first[i] + kSlice * e];
       strut_accumulate<FastMath>(F, p, v, ld3(S.x, S.y, S.z, se.nbr), ld3(S.vx, S.vy, S.vz, se.nbr), se.k, se.c, se.len);
     }
   const float H = fp.H, iH = 1.0f / H, iH2 = iH * iH, im = 1.0f / m, is = 1.0f / out_scale;
