@@ -102,6 +102,91 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
   finish_step<M>(A, P, i, m, p, v, F, pin);
 }
 
+// The step of one patch of a general mesh per workgroup (the operations of face_kernel and
+// csr_kernel, in the same order per vertex):
+//   phase 0  stage the states of the patch and its halo in LDS with one pass of loads;
+//   phase 1  every face with a corner in the patch: drag + lift, and for each patch corner its
+//            share (angle / PI) * (drag + lift) into that vertex's LDS share slots (a face on the
+//            patch border is computed by each patch that holds one of its corners);
+//   phase 2  every patch vertex: struts from LDS in strut order, its face shares in face order,
+//            integrate, collide, pin, store.
+template <class M>
+__global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, const PatchRange* __restrict__ patches) {
+  __shared__ float s_st[6][kPatchLocal];
+  __shared__ float s_sh[3][kPatchFaces * kBlock];
+  const PatchRange R = patches[blockIdx.x];
+  const uint32_t np = R.end - R.begin, nl = np + R.n_halo;
+  const SoA& S = A.in;
+  const StepParams P = A.params[R.mesh];
+  for (uint32_t e = threadIdx.x; e < nl; e += kBlock) {
+    const uint32_t g = e < np ? R.begin + e : T.halo[R.halo_begin + (e - np)];
+    s_st[0][e] = S.x[g]; s_st[1][e] = S.y[g]; s_st[2][e] = S.z[g];
+    s_st[3][e] = S.vx[g]; s_st[4][e] = S.vy[g]; s_st[5][e] = S.vz[g];
+  }
+  __syncthreads();
+  auto Pl = [&](uint32_t l) { return f3{s_st[0][l], s_st[1][l], s_st[2][l]}; };
+  auto Vl = [&](uint32_t l) { return f3{s_st[3][l], s_st[4][l], s_st[5][l]}; };
+  for (uint32_t q = threadIdx.x; q < R.n_faces; q += kBlock) {
+    const PatchFace pf = T.patch_faces[R.face_begin + q];
+    const f3 p0 = Pl(pf.v[0]), p1 = Pl(pf.v[1]), p2 = Pl(pf.v[2]);
+    const f3 sum = face_force<M>(p0, p1, p2, Vl(pf.v[0]), Vl(pf.v[1]), Vl(pf.v[2]), P);
+    const f3 c[3] = {p0, p1, p2};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (pf.share[k] == kNoShare) continue;
+      const float w = corner_weight<M>(c[k], c[(k + 1) % 3], c[(k + 2) % 3]);
+      s_sh[0][pf.share[k]] = w * sum.x;
+      s_sh[1][pf.share[k]] = w * sum.y;
+      s_sh[2][pf.share[k]] = w * sum.z;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x >= np) return;
+  const uint32_t t = threadIdx.x, i = R.begin + t;
+  const f3 p = Pl(t), v = Vl(t);
+  const float m = P.uniform_mass ? P.mass : A.mass[i];
+  f3 F = A.force_init ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
+  F = add(F, P.gravity);  // apply_external_point_forces (:527-531)
+  // apply_strut_forces (:533-551), in strut order
+  {
+    const StrutEnt* ent = T.s_ent + T.s_first[i];
+    const uint16_t* loc = T.s_loc + T.s_first[i];
+    const uint32_t ns = T.s_cnt[i];
+    for (uint32_t e = 0; e < ns; e += 4) {
+      StrutEnt se[4];
+      uint32_t l[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t k = kSlice * (e + j < ns ? e + j : e);
+        se[j] = ent[k];
+        l[j] = loc[k];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        if (e + j < ns) strut_accumulate<M>(F, p, v, Pl(l[j]), Vl(l[j]), se[j].k, se[j].c, se[j].len);
+    }
+  }
+  // apply_torsional_forces, when the step includes it (see csr_kernel)
+  if (T.t_row != nullptr) {
+    uint32_t e = T.t_row[i];
+    const uint32_t ee = T.t_row[i + 1];
+    if (e < ee) {
+      f3 ts = f3{0.0f, 0.0f, 0.0f};
+      for (; e < ee; ++e) {
+        const float4 hf = T.hinge_f[T.t_ent[e]];
+        ts = add(ts, f3{hf.x, hf.y, hf.z});
+      }
+      F = add(F, ts);
+    }
+  }
+  // apply_face_forces (:629-654): this vertex's face shares, in face order
+  for (uint32_t j = 0, nf = T.f_cnt[i]; j < nf; ++j)
+    F = add(F, f3{s_sh[0][j * kBlock + t], s_sh[1][j * kBlock + t], s_sh[2][j * kBlock + t]});
+  const bool pin = pinned(A.pins, i);
+  if (pin) F = f3{0.0f, 0.0f, 0.0f};  // (:522-524)
+  finish_step<M>(A, P, i, m, p, v, F, pin);
+}
+
 // Drag + lift (springy_mesh.rs:631-646) and the three corner weights angle / PI (:647-649) of every
 // face of the CSR meshes, once per face; corner k's share weight_k * (drag + lift) (the product of
 // :650-652) goes to its vertex's face entry.
@@ -277,6 +362,14 @@ hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blo
     if (mode == MODE_STEP) csr_kernel<FastMath, MODE_STEP><<<g, b, 0, s>>>(a, t, blocks);
     else csr_kernel<FastMath, MODE_FORCES><<<g, b, 0, s>>>(a, t, blocks);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_patches(const StepArgs& a, const CsrView& t, const PatchRange* patches, uint32_t n_patches,
+                          bool exact, hipStream_t s) {
+  if (n_patches == 0) return hipSuccess;
+  if (exact) patch_kernel<ExactMath><<<dim3(n_patches), dim3(kBlock), 0, s>>>(a, t, patches);
+  else patch_kernel<FastMath><<<dim3(n_patches), dim3(kBlock), 0, s>>>(a, t, patches);
   return hipGetLastError();
 }
 

@@ -27,6 +27,24 @@ struct HingeEnt {
   float k, c, rest, pad;
 };
 
+// One workgroup of the patch kernel: device vertices [begin, end) of mesh `mesh` (a connected patch
+// of at most kBlock vertices), its halo, halo[halo_begin .. halo_begin + n_halo) (every vertex
+// outside the patch that a patch vertex's struts or faces reach; patch plus halo is at most
+// kPatchLocal), and its faces, patch_faces[face_begin .. face_begin + n_faces) (every face with a
+// corner in the patch).
+struct PatchRange {
+  uint32_t mesh, begin, end, halo_begin, n_halo, face_begin, n_faces, pad;
+};
+constexpr uint32_t kPatchLocal = 512;
+// A face of a patch: its corners' indices in the patch's LDS copy, and for each corner that is a
+// patch vertex t, the LDS slot of its share, j * kBlock + t for the face's rank j among t's faces
+// in face-index order (kNoShare: a halo corner).  A patch vertex has at most kPatchFaces faces.
+struct PatchFace {
+  uint16_t v[3], share[3];
+};
+constexpr uint16_t kNoShare = 0xffff;
+constexpr uint32_t kPatchFaces = 8;
+
 // Vertex-centric incidence lists of a packed set of meshes (global device vertex and face ids), in
 // a sliced layout: the vertices of one mesh are cut into slices of kSlice consecutive ids (one
 // wavefront), and entry j of a vertex is at first[v] + kSlice * j, so that one load instruction of
@@ -43,6 +61,11 @@ struct CsrView {
   // per face entry: that corner's share of its face's drag + lift, (angle / PI) * (drag + lift),
   // written by the face kernel at f_slot
   float4* face_c;
+  // patch kernel: per strut entry the neighbour's index in its patch's LDS copy (the patch's own
+  // vertices first, then its halo), and the halo vertex ids of all patches
+  const uint16_t* s_loc;
+  const uint32_t* halo;
+  const PatchFace* patch_faces;
   // torsional hinges in the step (nullptr: none): per vertex its (hinge << 2 | role) entries in
   // strut order (plain CSR, N+1 row pointers), and the 4 forces per hinge written by the hinge kernel
   const uint32_t* t_row;
@@ -88,6 +111,10 @@ constexpr int kTileCols[N_TILE_SHAPES] = {15, 31, 16};
 
 hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
                       bool exact, hipStream_t s);
+// The step (MODE_STEP) of general meshes one patch per workgroup, the neighbour states read from
+// the patch's LDS copy; the same results as launch_csr.
+hipError_t launch_patches(const StepArgs& a, const CsrView& t, const PatchRange* patches, uint32_t n_patches,
+                          bool exact, hipStream_t s);
 // Drag + lift and corner weights of the faces in `face_blocks` (BlockRange over global face ids),
 // for the CSR kernel that runs after it in the same stream.
 hipError_t launch_faces(const StepArgs& a, const CsrView& t, const BlockRange* face_blocks, uint32_t n_blocks,

@@ -89,25 +89,39 @@ struct MeshSlot {
   // Device storage order of a general mesh's vertices (empty: the reference order, as for cloths):
   // perm[d] = the reference index stored at device slot vbase + d, pos = its inverse.
   std::vector<uint32_t> perm, pos;
+  // device-local starts of the mesh's vertex blocks (one workgroup each), ending with n: patches of
+  // at most kBlock connected vertices for general meshes, kBlock strides otherwise
+  std::vector<uint32_t> bstart;
+  bool patch_ok = false;  // every patch's vertices and halo fit the patch kernel's LDS
   uint64_t dev(uint64_t i) const { return vbase + (pos.empty() ? i : pos[i]); }
   uint64_t ref(uint64_t d) const { return perm.empty() ? d : perm[d]; }
 };
 
-// Reverse Cuthill-McKee order of a mesh's strut graph: breadth-first from a lowest-degree vertex,
+// Strut adjacency of a mesh (CSR over reference vertex ids).
+struct Adjacency {
+  std::vector<uint32_t> row, adj;
+  uint32_t deg(uint32_t v) const { return row[v + 1] - row[v]; }
+};
+Adjacency strut_adjacency(const fp::HostMesh& h) {
+  Adjacency A;
+  A.row.assign(h.n + 1, 0);
+  A.adj.resize(2 * h.n_struts());
+  for (uint64_t k = 0; k < h.n_struts(); ++k) { A.row[h.s_v0[k] + 1]++; A.row[h.s_v1[k] + 1]++; }
+  for (uint64_t i = 0; i < h.n; ++i) A.row[i + 1] += A.row[i];
+  std::vector<uint32_t> fill(A.row.begin(), A.row.end() - 1);
+  for (uint64_t k = 0; k < h.n_struts(); ++k) {
+    A.adj[fill[h.s_v0[k]]++] = h.s_v1[k];
+    A.adj[fill[h.s_v1[k]]++] = h.s_v0[k];
+  }
+  return A;
+}
+
+// Reverse Cuthill-McKee order of a strut graph: breadth-first from a lowest-degree vertex,
 // neighbours in ascending degree, each connected component in turn, the whole order reversed.
-// Neighbours then sit within a narrow band of indices, so the per-vertex gathers of one
-// workgroup read few cache lines, most of them shared with the neighbouring workgroups.
-std::vector<uint32_t> rcm_order(const fp::HostMesh& h) {
-  const uint64_t n = h.n;
-  std::vector<uint32_t> row(n + 1, 0), adj(2 * h.n_struts()), fill;
-  for (uint64_t k = 0; k < h.n_struts(); ++k) { row[h.s_v0[k] + 1]++; row[h.s_v1[k] + 1]++; }
-  for (uint64_t i = 0; i < n; ++i) row[i + 1] += row[i];
-  fill.assign(row.begin(), row.end() - 1);
-  for (uint64_t k = 0; k < h.n_struts(); ++k) { adj[fill[h.s_v0[k]]++] = h.s_v1[k]; adj[fill[h.s_v1[k]]++] = h.s_v0[k]; }
-  auto deg = [&](uint32_t v) { return row[v + 1] - row[v]; };
+std::vector<uint32_t> rcm_order(const Adjacency& A, uint64_t n) {
   std::vector<uint32_t> by_deg(n), order;
   for (uint64_t i = 0; i < n; ++i) by_deg[i] = (uint32_t)i;
-  std::stable_sort(by_deg.begin(), by_deg.end(), [&](uint32_t a, uint32_t b) { return deg(a) < deg(b); });
+  std::stable_sort(by_deg.begin(), by_deg.end(), [&](uint32_t a, uint32_t b) { return A.deg(a) < A.deg(b); });
   std::vector<uint8_t> seen(n, 0);
   std::vector<uint32_t> nb;
   order.reserve(n);
@@ -119,14 +133,42 @@ std::vector<uint32_t> rcm_order(const fp::HostMesh& h) {
     for (size_t q = head; q < order.size(); ++q) {
       const uint32_t v = order[q];
       nb.clear();
-      for (uint32_t e = row[v]; e < row[v + 1]; ++e)
-        if (!seen[adj[e]]) { seen[adj[e]] = 1; nb.push_back(adj[e]); }
-      std::stable_sort(nb.begin(), nb.end(), [&](uint32_t a, uint32_t b) { return deg(a) < deg(b); });
+      for (uint32_t e = A.row[v]; e < A.row[v + 1]; ++e)
+        if (!seen[A.adj[e]]) { seen[A.adj[e]] = 1; nb.push_back(A.adj[e]); }
+      std::stable_sort(nb.begin(), nb.end(), [&](uint32_t a, uint32_t b) { return A.deg(a) < A.deg(b); });
       order.insert(order.end(), nb.begin(), nb.end());
     }
   }
   std::reverse(order.begin(), order.end());
   return order;
+}
+
+// Device order of a general mesh: patches of at most kBlock vertices grown breadth-first over the
+// strut graph from seeds taken in reverse Cuthill-McKee order, each patch's vertices in the order
+// they were reached.  A patch is a compact piece of surface, so its halo (the neighbours outside
+// it that its struts reach) is small and one workgroup can stage patch and halo in LDS; the
+// sweep order keeps successive patches adjacent.  perm[d] = the reference vertex at device slot d,
+// bstart = the patches' first slots, ending with n.
+void patch_order(const fp::HostMesh& h, std::vector<uint32_t>& perm, std::vector<uint32_t>& bstart) {
+  const Adjacency A = strut_adjacency(h);
+  const std::vector<uint32_t> seeds = rcm_order(A, h.n);
+  std::vector<uint8_t> taken(h.n, 0);
+  perm.clear();
+  perm.reserve(h.n);
+  bstart.clear();
+  for (uint32_t seed : seeds) {
+    if (taken[seed]) continue;
+    const size_t head = perm.size();
+    bstart.push_back((uint32_t)head);
+    taken[seed] = 1;
+    perm.push_back(seed);
+    for (size_t q = head; q < perm.size() && perm.size() - head < fp::kBlock; ++q) {
+      const uint32_t v = perm[q];
+      for (uint32_t e = A.row[v]; e < A.row[v + 1] && perm.size() - head < fp::kBlock; ++e)
+        if (!taken[A.adj[e]]) { taken[A.adj[e]] = 1; perm.push_back(A.adj[e]); }
+    }
+  }
+  bstart.push_back((uint32_t)h.n);
 }
 
 // Default tile shape for a cloth of this size: 16x16 (fastest measured on the 1024 x 1024 cloth,
@@ -151,6 +193,14 @@ struct fp_sim {
   DBuf<uint32_t> s_first, s_cnt, f_first, f_cnt, f_v, f_slot;
   DBuf<fp::StrutEnt> s_ent;
   DBuf<float4> face_c;
+  // patch kernel: per block of a general mesh its patch and halo (h_patches[b] for block b), the
+  // halo ids, per strut entry the neighbour's LDS index; the launch list of the meshes on it
+  std::vector<fp::PatchRange> h_patches;
+  DBuf<uint32_t> halo;
+  DBuf<uint16_t> s_loc;
+  DBuf<fp::PatchFace> patch_faces;
+  DBuf<fp::PatchRange> patch_list;
+  uint32_t n_patch_list = 0;
   // face blocks of the meshes on the CSR kernel (BlockRange over global face ids), all meshes'
   // face blocks (for the split API, indexed per mesh by fblk_begin/fblk_end)
   DBuf<fp::BlockRange> csr_face_blocks, all_face_blocks;
@@ -238,7 +288,8 @@ struct fp_sim {
 
   fp::CsrView csr(bool with_torsion = false) const {
     const bool t = (with_torsion || torsion_on) && n_hinges > 0;
-    return fp::CsrView{s_first.p, s_cnt.p, s_ent.p, f_first.p, f_cnt.p, f_v.p, f_slot.p, face_c.p,
+    return fp::CsrView{s_first.p, s_cnt.p, s_ent.p, f_first.p, f_cnt.p, f_v.p, f_slot.p, face_c.p, s_loc.p, halo.p,
+                       patch_faces.p,
                        t ? t_row.p : nullptr, t ? t_ent.p : nullptr, hinge_f.p};
   }
 };
@@ -316,6 +367,7 @@ std::vector<T> xcd_order(const std::vector<T>& t) {
 
 int build_block_lists(fp_sim* s) {
   std::vector<fp::BlockRange> g, c, cf, sb, sf;
+  std::vector<fp::PatchRange> pl;
   std::vector<fp::TileRange> by_shape[fp::N_TILE_SHAPES], all_tiles;
   for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
     MeshSlot& ms = s->meshes[mi];
@@ -333,17 +385,21 @@ int build_block_lists(fp_sim* s) {
       continue;
     }
     for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) {
-      (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
+      if (ms.kernel == MK_CSR && ms.patch_ok) pl.push_back(s->h_patches[b]);
+      else (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
       sb.push_back(s->h_blocks[b]);
     }
     for (uint32_t b = ms.fblk_begin; b < ms.fblk_end; ++b) {
-      if (ms.kernel == MK_CSR) cf.push_back(s->h_face_blocks[b]);
+      if (ms.kernel == MK_CSR && !ms.patch_ok) cf.push_back(s->h_face_blocks[b]);
       sf.push_back(s->h_face_blocks[b]);
     }
   }
   // the surrogate's generic path (face kernel + per-vertex update) covers every mesh that is not
   // on the tiled kernel
   c = xcd_order(c);
+  pl = xcd_order(pl);
+  s->n_patch_list = (uint32_t)pl.size();
+  HIP_TRY(s->patch_list.upload(pl.data(), pl.size()));
   cf = xcd_order(cf);
   sb = xcd_order(sb);
   sf = xcd_order(sf);
@@ -513,6 +569,7 @@ int run_steps(fp_sim* s, uint64_t n) {
     if (s->torsion_on)
       HIP_TRY(fp::launch_hinges(a, s->csr(), s->hinges.p, 0, s->n_hinges, exact, s->stream));
     HIP_TRY(fp::launch_csr(a, s->csr(), s->csr_blocks.p, s->n_csr_blocks, fp::MODE_STEP, exact, s->stream));
+    HIP_TRY(fp::launch_patches(a, s->csr(), s->patch_list.p, s->n_patch_list, exact, s->stream));
     s->cur ^= 1;
     s->forces_dirty = false;
   }
@@ -761,10 +818,13 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     Fent += 3 * slot.host.n_faces();
     refresh_mass(slot, slot.host.rec.data(), slot.host.n);
     if (slot.host.cloth.valid) slot.tile_shape = default_tile_shape(slot.host.cloth.rows, slot.host.cloth.cols);
-    else if (slot.host.n > 1) {
-      slot.perm = rcm_order(slot.host);
+    if (!slot.host.cloth.valid && slot.host.n > 1) {
+      patch_order(slot.host, slot.perm, slot.bstart);
       slot.pos.assign(slot.host.n, 0);
       for (uint64_t d = 0; d < slot.host.n; ++d) slot.pos[slot.perm[d]] = (uint32_t)d;
+    } else {
+      for (uint64_t b = 0; b < slot.host.n; b += fp::kBlock) slot.bstart.push_back((uint32_t)b);
+      slot.bstart.push_back((uint32_t)slot.host.n);
     }
     s->meshes.push_back(std::move(slot));
   }
@@ -818,24 +878,27 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
       }
       for (uint64_t k = 0; k < h.f_v.size(); ++k) fcnt[ms_.dev(h.f_v[k])]++;
     }
-    // slice g of a mesh (device ids vbase + [kSlice g, kSlice (g+1))) takes kSlice x (its largest
-    // count) entries; lane l's entry j is at the slice base + kSlice j + l
+    // slices: kSlice consecutive device ids from each block start (the wavefronts of the block's
+    // workgroup); a slice takes kSlice x (its largest count) entries, lane l's entry j at the slice
+    // base + kSlice j + l
     auto slices = [&](const std::vector<uint32_t>& cnt, std::vector<uint32_t>& first) -> uint64_t {
       uint64_t total = 0;
       for (auto& ms_ : s->meshes)
-        for (uint64_t g0 = 0; g0 < ms_.host.n; g0 += fp::kSlice) {
-          const uint64_t g1 = std::min<uint64_t>(ms_.host.n, g0 + fp::kSlice);
-          uint32_t mx = 0;
-          for (uint64_t d = g0; d < g1; ++d) mx = std::max(mx, cnt[ms_.vbase + d]);
-          for (uint64_t d = g0; d < g1; ++d) first[ms_.vbase + d] = (uint32_t)(total + (d - g0));
-          total += (uint64_t)fp::kSlice * mx;
-        }
+        for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b)
+          for (uint64_t g0 = ms_.bstart[b]; g0 < ms_.bstart[b + 1]; g0 += fp::kSlice) {
+            const uint64_t g1 = std::min<uint64_t>(ms_.bstart[b + 1], g0 + fp::kSlice);
+            uint32_t mx = 0;
+            for (uint64_t d = g0; d < g1; ++d) mx = std::max(mx, cnt[ms_.vbase + d]);
+            for (uint64_t d = g0; d < g1; ++d) first[ms_.vbase + d] = (uint32_t)(total + (d - g0));
+            total += (uint64_t)fp::kSlice * mx;
+          }
       return total;
     };
     const uint64_t n_sent = slices(scnt, sfirst), n_fent = slices(fcnt, ffirst);
     if (n_sent >= (1ull << 32) || n_fent >= (1ull << 32)) return fail(FP_EINVAL, "simulation too large for 32-bit ids");
     std::vector<fp::StrutEnt> sent(n_sent + 1, fp::StrutEnt{0, 1.0f, 0.0f, 0.0f});
     std::vector<uint32_t> fslot(3 * F), sfill(N, 0), ffill(N, 0);
+    std::vector<uint32_t> frank(3 * F);  // per face corner (reference face order): the face's rank among its vertex's faces
     for (auto& ms_ : s->meshes) {
       const auto& h = ms_.host;
       for (uint64_t k = 0; k < h.n_struts(); ++k) {
@@ -851,6 +914,7 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
         uint32_t m = 0xffffffffu;
         for (uint32_t j = 0; j < 3; ++j) {
           uint32_t v = (uint32_t)ms_.dev(h.f_v[3 * t + j]);
+          frank[ms_.fbase * 3 + 3 * t + j] = ffill[v];
           slot3[3 * t + j] = ffirst[v] + fp::kSlice * ffill[v]++;
           m = std::min(m, v);
         }
@@ -864,6 +928,73 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
           fslot[3 * (ms_.fbase + q) + j] = slot3[3 * (uint64_t)at[q] + j];
         }
     }
+    // patches of the general meshes (block b of every mesh is h_patches[b]): halo, faces, LDS
+    // indices.  A mesh with a patch whose vertices and halo exceed kPatchLocal, or with a vertex of
+    // more than kPatchFaces faces, stays on the CSR kernel.
+    std::vector<uint16_t> sloc(n_sent + 1, 0);
+    std::vector<uint32_t> halo;
+    std::vector<fp::PatchFace> pfaces;
+    std::vector<int32_t> mark(N, -1);
+    for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
+      auto& ms_ = s->meshes[mi];
+      const auto& h = ms_.host;
+      ms_.patch_ok = !ms_.perm.empty();
+      std::vector<uint32_t> vf_row, vf;  // reference faces of each device-local vertex
+      if (ms_.patch_ok) {
+        vf_row.assign(h.n + 1, 0);
+        for (uint64_t k = 0; k < h.f_v.size(); ++k) vf_row[ms_.pos[h.f_v[k]] + 1]++;
+        for (uint64_t d = 0; d < h.n; ++d) vf_row[d + 1] += vf_row[d];
+        vf.resize(h.f_v.size());
+        std::vector<uint32_t> fill(vf_row.begin(), vf_row.end() - 1);
+        for (uint64_t k = 0; k < h.f_v.size(); ++k) vf[fill[ms_.pos[h.f_v[k]]]++] = (uint32_t)(k / 3);
+        for (uint64_t i = ms_.vbase; i < ms_.vbase + h.n; ++i)
+          if (fcnt[i] > fp::kPatchFaces) ms_.patch_ok = false;
+      }
+      std::vector<uint8_t> fseen(ms_.patch_ok ? h.n_faces() : 0, 0);
+      for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b) {
+        const uint32_t b0 = (uint32_t)ms_.vbase + ms_.bstart[b], b1 = (uint32_t)ms_.vbase + ms_.bstart[b + 1];
+        fp::PatchRange pr{mi, b0, b1, (uint32_t)halo.size(), 0u, (uint32_t)pfaces.size(), 0u, 0u};
+        if (ms_.patch_ok) {
+          const size_t h0 = halo.size(), q0 = pfaces.size();
+          auto local = [&](uint32_t g) -> uint32_t {  // LDS index of device vertex g (adds it to the halo)
+            if (g >= b0 && g < b1) return g - b0;
+            if (mark[g] < 0) { mark[g] = (int32_t)(halo.size() - h0); halo.push_back(g); }
+            return (b1 - b0) + (uint32_t)mark[g];
+          };
+          for (uint32_t i = b0; i < b1; ++i)
+            for (uint32_t j = 0; j < scnt[i]; ++j) {
+              const uint32_t e = sfirst[i] + fp::kSlice * j;
+              sloc[e] = (uint16_t)std::min<uint32_t>(local(sent[e].nbr), fp::kPatchLocal - 1);
+            }
+          std::vector<uint32_t> fl;
+          for (uint32_t i = b0; i < b1; ++i)
+            for (uint32_t k = vf_row[i - ms_.vbase]; k < vf_row[i - ms_.vbase + 1]; ++k)
+              if (!fseen[vf[k]]) { fseen[vf[k]] = 1; fl.push_back(vf[k]); }
+          for (uint32_t f : fl) {
+            fp::PatchFace pf{};
+            for (int c = 0; c < 3; ++c) {
+              const uint32_t g = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)f + c]);
+              pf.v[c] = (uint16_t)std::min<uint32_t>(local(g), fp::kPatchLocal - 1);
+              pf.share[c] = (g >= b0 && g < b1)
+                                ? (uint16_t)(frank[3 * (ms_.fbase + f) + c] * fp::kBlock + (g - b0))
+                                : fp::kNoShare;
+            }
+            pfaces.push_back(pf);
+          }
+          for (uint32_t f : fl) fseen[f] = 0;
+          pr.n_halo = (uint32_t)(halo.size() - h0);
+          pr.n_faces = (uint32_t)(pfaces.size() - q0);
+          if ((b1 - b0) + pr.n_halo > fp::kPatchLocal) ms_.patch_ok = false;
+          for (size_t k = h0; k < halo.size(); ++k) mark[halo[k]] = -1;
+        }
+        s->h_patches.push_back(pr);
+      }
+    }
+    pfaces.push_back(fp::PatchFace{});
+    HIP_TRY(s->patch_faces.upload(pfaces.data(), pfaces.size()));
+    HIP_TRY(s->s_loc.upload(sloc.data(), sloc.size()));
+    halo.push_back(0);
+    HIP_TRY(s->halo.upload(halo.data(), halo.size()));
     HIP_TRY(s->s_first.upload(sfirst.data(), sfirst.size()));
     HIP_TRY(s->s_cnt.upload(scnt.data(), scnt.size()));
     HIP_TRY(s->s_ent.upload(sent.data(), sent.size()));
@@ -936,9 +1067,9 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
       c.tk = d.tk; c.tc = d.tc; c.sk = d.sk; c.sc = d.sc; c.bk = d.bk; c.bc = d.bc;
       cv.push_back(c);
       ms_.blk_begin = (uint32_t)s->h_blocks.size();
-      for (uint64_t b = 0; b < ms_.host.n; b += fp::kBlock)
-        s->h_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.vbase + b),
-                                             (uint32_t)(ms_.vbase + std::min<uint64_t>(ms_.host.n, b + fp::kBlock)), 0u});
+      for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b)
+        s->h_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.vbase + ms_.bstart[b]),
+                                             (uint32_t)(ms_.vbase + ms_.bstart[b + 1]), 0u});
       ms_.blk_end = (uint32_t)s->h_blocks.size();
       ms_.fblk_begin = (uint32_t)s->h_face_blocks.size();
       const uint64_t nf = ms_.host.n_faces();
