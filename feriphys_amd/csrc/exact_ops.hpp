@@ -17,4 +17,15 @@ FP_HD float div3_rn(float x) {
   return q0 == 0.0f ? q0 : __builtin_fmaf(r, third, q0);
 }
 
+// x / PI (PI = std::f32::consts::PI = 0x40490fdb), correctly rounded: for x >= 2^-100 the product
+// with RN(1/PI) corrected once by the exact remainder fma(-q0, PI, x) (the same method as div3_rn;
+// below 2^-104 the remainder is no longer exact), otherwise (tiny, zero, negative, NaN) the
+// division itself.
+FP_HD float div_pi_rn(float x) {
+  const float pi = 3.14159274101257324f, inv_pi = 0.31830987334251404f;  // 0x40490fdb, RN(1/PI) = 0x3ea2f983
+  if (!(x >= 0x1p-100f)) return x / pi;
+  const float q0 = x * inv_pi;
+  return __builtin_fmaf(__builtin_fmaf(-q0, pi, x), inv_pi, q0);
+}
+
 }  // namespace fp

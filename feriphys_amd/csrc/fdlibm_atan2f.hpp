@@ -9,7 +9,10 @@
 // polynomial coefficients are fdlibm's double-precision ones rounded to f32; atan(c) hi parts are
 // truncated to f32 and lo = round(atan(c) - hi).  Must be compiled without FMA contraction.
 //
-// Checked bit for bit against the host C library's atan2f on 20M inputs (tests).
+// fdlibm_atan2f_ref follows fdlibm's control flow; fdlibm_atan2f computes the same bits without its
+// data-dependent branches (one reduction division with selected constants; see below), which is
+// what a wavefront wants when its lanes fall in different reduction intervals.  Both are checked
+// bit for bit against the host C library's atan2f (tests).
 #pragma once
 
 #include <stdint.h>
@@ -81,7 +84,7 @@ FP_HD float fdlibm_atanf(float x) {
   return hx < 0 ? -r : r;
 }
 
-FP_HD float fdlibm_atan2f(float y, float x) {
+FP_HD float fdlibm_atan2f_ref(float y, float x) {
   const float tiny = 1.0e-30f;
   const float pi_o_4 = fdl_from_bits(0x3f490fdbu), pi_o_2 = fdl_from_bits(0x3fc90fdbu),
               pi = fdl_from_bits(0x40490fdbu), pi_lo = fdl_from_bits(0xb3bbbd2eu);
@@ -119,6 +122,52 @@ FP_HD float fdlibm_atan2f(float y, float x) {
   if (m == 1) return -z;
   if (m == 2) return pi - (z - pi_lo);
   return (z - pi_lo) - pi;
+}
+
+// fdlibm_atan2f_ref without data-dependent branches, for finite non-zero y and x (other inputs take
+// the reference path).  Why each fdlibm branch can go:
+//   x == 1 calls atanf(y) directly: the general path computes atanf(|y / 1|) and the same sign;
+//   |y/x| > 2^60 and (x < 0, |y/x| < 2^-60) return constants that the general path rounds to as
+//   well (atan(t >= 2^25) is pi/2 in f32; a tiny z vanishes against pi_lo);
+//   the four reduction intervals share one form, atan(t) = hi + atan((t A - B) / (A + t B)) with
+//   (A, B) = (1, 0), (2, 1), (1, 1), (1, 1.5), (0, 1) for t < 7/16, < 11/16, < 19/16, < 39/16,
+//   above: each numerator and denominator then rounds exactly like fdlibm's (t*1 = t, t*0 = 0,
+//   0 + t = t), and for the first interval hi = lo = 0 gives 0 - ((u S - 0) - u) = u - u S.
+FP_HD float fdlibm_atan2f(float y, float x) {
+  const uint32_t ix = fdl_bits(x) & 0x7fffffffu, iy = fdl_bits(y) & 0x7fffffffu;
+  if (ix - 1u >= 0x7f7fffffu || iy - 1u >= 0x7f7fffffu) return fdlibm_atan2f_ref(y, x);  // 0, inf, NaN
+  const float pi = fdl_from_bits(0x40490fdbu), pi_lo = fdl_from_bits(0xb3bbbd2eu);
+  const float t = fdl_abs(y / x);
+  const uint32_t it = fdl_bits(t);
+  const bool c0 = it >= 0x3ee00000u, c1 = it >= 0x3f300000u, c2 = it >= 0x3f980000u, c3 = it >= 0x401c0000u;
+  const float A = c3 ? 0.0f : (c0 && !c1) ? 2.0f : 1.0f;
+  const float B = !c0 ? 0.0f : (c2 && !c3) ? 1.5f : 1.0f;
+  const float hi = !c0   ? 0.0f
+                   : !c1 ? fdl_from_bits(0x3eed6338u)
+                   : !c2 ? fdl_from_bits(0x3f490fdau)
+                   : !c3 ? fdl_from_bits(0x3f7b985eu)
+                         : fdl_from_bits(0x3fc90fdau);
+  const float lo = !c0   ? 0.0f
+                   : !c1 ? fdl_from_bits(0x31ac376au)
+                   : !c2 ? fdl_from_bits(0x33222169u)
+                   : !c3 ? fdl_from_bits(0x33140fb5u)
+                         : fdl_from_bits(0x33a22169u);
+  const float u = (t * A - B) / (A + t * B);
+  const float a0 = fdl_from_bits(0x3eaaaaabu), a1 = fdl_from_bits(0xbe4ccccdu), a2 = fdl_from_bits(0x3e124925u),
+              a3 = fdl_from_bits(0xbde38e38u), a4 = fdl_from_bits(0x3dba2e6eu), a5 = fdl_from_bits(0xbd9d8795u),
+              a6 = fdl_from_bits(0x3d886b35u), a7 = fdl_from_bits(0xbd6ef16bu), a8 = fdl_from_bits(0x3d4bda59u),
+              a9 = fdl_from_bits(0xbd15a221u), a10 = fdl_from_bits(0x3c8569d7u);
+  const float z = u * u;
+  const float w = z * z;
+  const float s1 = z * (a0 + w * (a2 + w * (a4 + w * (a6 + w * (a8 + w * a10)))));
+  const float s2 = w * (a1 + w * (a3 + w * (a5 + w * (a7 + w * a9))));
+  float r = hi - ((u * (s1 + s2) - lo) - u);
+  r = it < 0x31000000u ? t : r;                                // |t| < 2^-29: atan(t) = t
+  r = it >= 0x4c000000u ? fdl_from_bits(0x3fc90fdbu) : r;      // t >= 2^25: atanhi[3] + atanlo[3]
+  const bool yneg = fdl_bits(y) >> 31, xneg = fdl_bits(x) >> 31;
+  if (!xneg) return yneg ? -r : r;
+  const float q = yneg ? (r - pi_lo) - pi : pi - (r - pi_lo);
+  return q;
 }
 
 }  // namespace fp

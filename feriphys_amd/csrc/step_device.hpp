@@ -126,7 +126,7 @@ __device__ __forceinline__ float corner_weight(f3 pa, f3 pnext, f3 pprev) {
   f3 b = sub(pnext, pa);
   f3 axb = cross(a, b);
   float th = M::atan2_(M::sqrt_(M::dot(axb, axb)), M::dot(a, b));
-  return M::div(th, 3.14159274101257324f);  // Rad / Rad(PI), PI = std::f32::consts::PI
+  return M::div_pi(th);  // Rad / Rad(PI), PI = std::f32::consts::PI
 }
 
 // Forces of one torsional hinge (springy_mesh.rs:555-610, the reference's operation order): hinge

@@ -51,6 +51,7 @@ struct ExactMath {
   __device__ static float rcp(float x) { return 1.0f / x; }
   __device__ static float div(float a, float b) { return a / b; }
   __device__ static float atan2_(float y, float x) { return fdlibm_atan2f(y, x); }
+  __device__ static float div_pi(float x) { return div_pi_rn(x); }
   __device__ static float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
   __device__ static float mad(float a, float b, float c) { return a * b + c; }
 };
@@ -84,6 +85,7 @@ struct FastMath {
   __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
   __device__ static float div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
   __device__ static float atan2_(float y, float x) { return fast_atan2(y, x); }
+  __device__ static float div_pi(float x) { return div(x, 3.14159274101257324f); }
   __device__ static float dot(f3 a, f3 b) { return __builtin_fmaf(a.z, b.z, __builtin_fmaf(a.y, b.y, a.x * b.x)); }
   __device__ static float mad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 };

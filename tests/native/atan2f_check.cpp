@@ -1,5 +1,6 @@
-// Compares fp::fdlibm_atan2f (the device kernels' atan2) with the host C library's atan2f, which is
-// what the reference's f32::atan2 calls.  Prints the number of bit mismatches (NaN payloads aside).
+// Compares fp::fdlibm_atan2f (the device kernels' branch-free atan2) and fp::fdlibm_atan2f_ref
+// (fdlibm's control flow) with the host C library's atan2f, which is what the reference's
+// f32::atan2 calls.  Prints the number of bit mismatches (NaN payloads aside).
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -20,8 +21,9 @@ int main(int argc, char** argv) {
                            1e-30f, -1e-30f, 1e30f, -1e30f, 3.0f, -3.0f, 1e-45f, 16777216.0f};
   for (float y : special)
     for (float x : special) {
-      float a = std::atan2(y, x), b = fp::fdlibm_atan2f(y, x);
+      float a = std::atan2(y, x), b = fp::fdlibm_atan2f(y, x), c = fp::fdlibm_atan2f_ref(y, x);
       if (bf(a) != bf(b)) { ++mism; std::printf("special y=%a x=%a libm=%a fdlibm=%a\n", y, x, a, b); }
+      if (bf(a) != bf(c)) { ++mism; std::printf("special y=%a x=%a libm=%a fdlibm_ref=%a\n", y, x, a, c); }
     }
   for (long i = 0; i < n; ++i) {
     float y, x;
@@ -31,8 +33,9 @@ int main(int argc, char** argv) {
       y = (float)((r >> 11) * (1.0 / 9007199254740992.0)) * 4.0f;
       x = (float)((xr() >> 11) * (1.0 / 9007199254740992.0)) * 8.0f - 4.0f;
     }
-    float a = std::atan2(y, x), b = fp::fdlibm_atan2f(y, x);
+    float a = std::atan2(y, x), b = fp::fdlibm_atan2f(y, x), c = fp::fdlibm_atan2f_ref(y, x);
     if (bf(a) != bf(b) && !(std::isnan(a) && std::isnan(b))) ++mism;
+    if (bf(a) != bf(c) && !(std::isnan(a) && std::isnan(c))) ++mism;
   }
   std::printf("%ld\n", mism);
   return 0;

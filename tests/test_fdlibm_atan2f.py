@@ -1,6 +1,8 @@
 """The exact kernels' atan2 (feriphys_amd/csrc/fdlibm_atan2f.hpp) rounds like the host C
 library's atan2f, which the reference's f32::atan2 calls: zero bit mismatches on special values
-and on 20M random inputs (a third of them arbitrary bit patterns)."""
+and on 20M random inputs (a third of them arbitrary bit patterns), for both the branch-free form
+the kernels use and fdlibm's own control flow; and the division-free exact helpers equal the
+division on every finite input."""
 import os
 import subprocess
 
@@ -23,5 +25,15 @@ def test_div3_rn_exhaustive(tmp_path):
     exe = str(tmp_path / "div3_check")
     subprocess.run(["g++", "-O2", "-std=c++17", "-mfma", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
                     os.path.join(HERE, "native", "div3_check.cpp"), "-o", exe], check=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.strip()
+    assert out == "0"
+
+
+def test_div_pi_rn_exhaustive(tmp_path):
+    """div_pi_rn (the corner weight angle / PI without a division) equals x / PI on all finite f32
+    inputs."""
+    exe = str(tmp_path / "div_pi_check")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-mfma", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+                    os.path.join(HERE, "native", "div_pi_check.cpp"), "-o", exe], check=True)
     out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.strip()
     assert out == "0"
