@@ -6,11 +6,13 @@ include/feriphys_gpu.h); this package is a thin ctypes mirror of the reference's
 from ._lib import (FP_EULER, FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_KERNEL_GRID_SIMPLE, FP_MATH_EXACT,
                    FP_MATH_FAST,
                    FP_PLANE_XY, FP_PLANE_XZ, FP_RK4, FpError, lib)
-from .springy import (Cloth, CollidableMesh, Config, Integration, Simulation, SpringyMesh, config_default)
+from .springy import (Cloth, CollidableMesh, Config, Integration, Simulation, SpringyMesh, config_default,
+                      selftest_exact_math)
 from .surrogate import Surrogate, SurrogateConfig, SurrogateStats
 
 __all__ = [
     "Cloth", "CollidableMesh", "Config", "Integration", "Simulation", "SpringyMesh", "config_default", "FpError",
+    "selftest_exact_math",
     "Surrogate", "SurrogateConfig", "SurrogateStats",
     "lib", "FP_EULER", "FP_RK4", "FP_PLANE_XY", "FP_PLANE_XZ", "FP_KERNEL_AUTO", "FP_KERNEL_CSR", "FP_KERNEL_GRID",
     "FP_KERNEL_GRID_SIMPLE",

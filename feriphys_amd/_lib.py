@@ -57,6 +57,7 @@ FP_SURROGATE_FEATURES = 72
 SIGNATURES = {
     "fp_last_error": (C.c_char_p, []),
     "fp_version": (C.c_char_p, []),
+    "fp_selftest_exact_math": (C.c_int, [C.POINTER(C.c_uint64)]),
     "fp_config_default": (None, [C.POINTER(FpConfig)]),
     "fp_mesh_new": (C.c_int, [_FP, _U64, _U64P, _U64, C.c_float, C.c_float, C.c_float, C.POINTER(FpTorsionConfig),
                               C.POINTER(FpStrutOverride), _U64, _PP]),

@@ -17,6 +17,21 @@ FP_HD float div3_rn(float x) {
   return q0 == 0.0f ? q0 : __builtin_fmaf(r, third, q0);
 }
 
+// 1.0f / x, correctly rounded.  On gfx950 for |x| in [2^-126, 2^126]: the hardware reciprocal
+// (within 1 ulp) refined once by Newton's step with the exact residual fma(-x, r, 1); checked on
+// every such input against the division (fp_selftest_exact_math).  Other inputs (zero, subnormal,
+// huge, inf, NaN) and host code take the division.
+FP_HD float rcp_rn(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const float ax = __builtin_fabsf(x);
+  if (ax >= 0x1p-126f && ax <= 0x1p126f) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+  }
+#endif
+  return 1.0f / x;
+}
+
 // x / PI (PI = std::f32::consts::PI = 0x40490fdb), correctly rounded: for x >= 2^-100 the product
 // with RN(1/PI) corrected once by the exact remainder fma(-q0, PI, x) (the same method as div3_rn;
 // below 2^-104 the remainder is no longer exact), otherwise (tiny, zero, negative, NaN) the

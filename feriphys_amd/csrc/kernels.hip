@@ -365,6 +365,34 @@ hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blo
   return hipGetLastError();
 }
 
+// Every finite f32 x (grid-stride over all bit patterns): the exact kernels' division-free forms
+// against the operations they stand for, bad[0] rcp_rn(x) vs 1 / x, bad[1] div_pi_rn(x) vs x / PI,
+// bad[2] div3_rn(x) vs x / 3, bad[3] the branch-free atan2 vs fdlibm's control flow at y = 1 and
+// y = -1.7 (NaN payloads aside).
+__global__ void __launch_bounds__(kBlock) selftest_kernel(unsigned long long* __restrict__ bad) {
+  unsigned long long b[4] = {0, 0, 0, 0};
+  auto bits = [](float f) { return __float_as_uint(f); };
+  for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < (1ull << 32); k += (uint64_t)gridDim.x * kBlock) {
+    const float x = __uint_as_float((uint32_t)k);
+    if (!__builtin_isfinite(x)) continue;
+    b[0] += bits(rcp_rn(x)) != bits(1.0f / x);
+    b[1] += bits(div_pi_rn(x)) != bits(x / 3.14159274101257324f);
+    b[2] += bits(div3_rn(x)) != bits(x / 3.0f);
+    const float ys[2] = {1.0f, -1.7f};
+    for (int j = 0; j < 2; ++j) {
+      const float a = fdlibm_atan2f(ys[j], x), r = fdlibm_atan2f_ref(ys[j], x);
+      b[3] += bits(a) != bits(r) && !(__builtin_isnan(a) && __builtin_isnan(r));
+    }
+  }
+  for (int q = 0; q < 4; ++q)
+    if (b[q]) atomicAdd(&bad[q], b[q]);
+}
+
+hipError_t launch_selftest(unsigned long long* bad, hipStream_t s) {
+  selftest_kernel<<<dim3(4096), dim3(kBlock), 0, s>>>(bad);
+  return hipGetLastError();
+}
+
 hipError_t launch_patches(const StepArgs& a, const CsrView& t, const PatchRange* patches, uint32_t n_patches,
                           bool exact, hipStream_t s) {
   if (n_patches == 0) return hipSuccess;

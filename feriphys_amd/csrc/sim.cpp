@@ -619,6 +619,19 @@ extern "C" {
 
 const char* fp_last_error(void) { return g_err.c_str(); }
 const char* fp_version(void) { return "feriphys_gpu 0.1 gfx950"; }
+
+int fp_selftest_exact_math(uint64_t mismatches[4]) {
+  if (!mismatches) return fail(FP_EINVAL, "mismatches is NULL");
+  DBuf<unsigned long long> d;
+  HIP_TRY(d.alloc(4));
+  HIP_TRY(hipMemset(d.p, 0, 4 * sizeof(unsigned long long)));
+  HIP_TRY(fp::launch_selftest(d.p, nullptr));
+  HIP_TRY(hipDeviceSynchronize());
+  unsigned long long h[4];
+  HIP_TRY(hipMemcpy(h, d.p, sizeof(h), hipMemcpyDeviceToHost));
+  for (int q = 0; q < 4; ++q) mismatches[q] = h[q];
+  return FP_OK;
+}
 void fp_config_default(fp_config* out) {
   if (out) *out = default_config();
 }

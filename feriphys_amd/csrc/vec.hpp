@@ -42,13 +42,13 @@ inline float h_magnitude(f3 v) { return sqrtf(dot(v, v)); }
 inline f3 h_normalize(f3 v) { return smul(1.0f / h_magnitude(v), v); }
 
 // ---- math policies for the kernels ------------------------------------------------------------
-// ExactMath: correctly rounded sqrt and division (the HIP default for f32, left on), the C
-// library's atan2f rounding (fdlibm_atan2f.hpp), no fused multiply-add: the reference's f32
-// results bit for bit.
+// ExactMath: correctly rounded sqrt, division and reciprocal (the HIP defaults for f32, or the
+// correctly rounded forms of exact_ops.hpp), the C library's atan2f rounding (fdlibm_atan2f.hpp),
+// no fused multiply-add in the reference's arithmetic: the reference's f32 results bit for bit.
 struct ExactMath {
   static constexpr bool kExact = true;
   __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
-  __device__ static float rcp(float x) { return 1.0f / x; }
+  __device__ static float rcp(float x) { return rcp_rn(x); }
   __device__ static float div(float a, float b) { return a / b; }
   __device__ static float atan2_(float y, float x) { return fdlibm_atan2f(y, x); }
   __device__ static float div_pi(float x) { return div_pi_rn(x); }

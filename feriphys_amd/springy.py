@@ -387,3 +387,11 @@ def config_default() -> Config:
     c = _lib.FpConfig()
     lib().fp_config_default(C.byref(c))
     return Config.from_c(c)
+
+
+def selftest_exact_math() -> dict:
+    """Runs fp_selftest_exact_math on the current GPU: over every finite f32, how often the exact
+    kernels' division-free forms differ from the operations they replace (all 0 expected)."""
+    out = (C.c_uint64 * 4)()
+    check(lib().fp_selftest_exact_math(out))
+    return {"rcp": int(out[0]), "div_pi": int(out[1]), "div3": int(out[2]), "atan2": int(out[3])}
