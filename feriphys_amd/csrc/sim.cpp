@@ -384,13 +384,16 @@ int build_block_lists(fp_sim* s) {
       ms.tile_end = (uint32_t)all_tiles.size();
       continue;
     }
+    // general meshes: the patch kernel where every patch fits it, unless FP_KERNEL_CSR asks for the
+    // one-thread-per-vertex CSR kernel (with the face kernel before it)
+    const bool on_patch = ms.kernel == MK_CSR && ms.patch_ok && s->kernel != FP_KERNEL_CSR;
     for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) {
-      if (ms.kernel == MK_CSR && ms.patch_ok) pl.push_back(s->h_patches[b]);
+      if (on_patch) pl.push_back(s->h_patches[b]);
       else (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
       sb.push_back(s->h_blocks[b]);
     }
     for (uint32_t b = ms.fblk_begin; b < ms.fblk_end; ++b) {
-      if (ms.kernel == MK_CSR && !ms.patch_ok) cf.push_back(s->h_face_blocks[b]);
+      if (ms.kernel == MK_CSR && !on_patch) cf.push_back(s->h_face_blocks[b]);
       sf.push_back(s->h_face_blocks[b]);
     }
   }

@@ -192,8 +192,12 @@ int fp_sim_apply_torsional_forces(fp_sim* sim, uint64_t mesh);
 int fp_sim_set_torsion(fp_sim* sim, int32_t enabled);
 
 /* Kernel selection.  FP_KERNEL_AUTO uses the LDS-tiled grid kernel for meshes built by Cloth::new
- * and left unchanged, the CSR kernel otherwise; FP_KERNEL_GRID_SIMPLE is the one-thread-per-vertex
- * grid kernel without LDS (kept as a cross-check).  EXACT kernels follow the reference's operation
+ * and left unchanged, and for other meshes the patch kernel (one workgroup per connected patch of
+ * up to 256 vertices, patch and halo staged in LDS, faces computed in the same pass) when every
+ * patch and halo fits it (at most 512 vertices, at most 8 faces per vertex), the CSR kernel (one
+ * thread per vertex, after a face kernel) otherwise.  FP_KERNEL_CSR puts every mesh on the CSR
+ * kernel; FP_KERNEL_GRID_SIMPLE is the one-thread-per-vertex grid kernel without LDS (kept as a
+ * cross-check).  All kernels return the same results.  EXACT kernels follow the reference's operation
  * order with no FMA contraction, correctly rounded division and square root and the C library's
  * atan2f rounding (bit-identical results); FAST kernels use fused multiply-adds and the hardware
  * reciprocal and square root (one-step forces within 1e-5 relative). */
