@@ -26,9 +26,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VERTEX_STEP = 48
 # HBM bytes per launch of the default step kernel (grid_tiled_kernel<ExactMath, 16, 16>) on the
-# 1024 x 1024 cloth, from rocprofv3 FETCH_SIZE (13,230 KiB x 2.000) and WRITE_SIZE (24,580 KiB x
+# 1024 x 1024 cloth, from rocprofv3 FETCH_SIZE (13,212 KiB x 2.000) and WRITE_SIZE (24,576 KiB x
 # 1.000), each scaled by a 256 MiB device copy's reading (profiles/grid_tiled_exact_cloth1024.md)
-MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13230 * 2.0 + 24580 * 1.0) * 1024}
+MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13212 * 2.0 + 24576 * 1.0) * 1024}
+# the same for patch_kernel<ExactMath> on the 64-icosphere workload (FETCH_SIZE 300,135 KiB x 2.000,
+# WRITE_SIZE 50,356 KiB x 1.000; profiles/batched_and_surfaces.md)
+MEASURED_TRAFFIC_SURFACES = {"exact": (300135 * 2.0 + 50356 * 1.0) * 1024}
 
 
 def main():
@@ -322,7 +325,7 @@ def bench_batched(args, np, fa, wl):
 
 
 def bench_surfaces(args, np, fa, wl):
-    """64 icospheres of uneven size (levels 4-7, 2,145,408 vertices) on the CSR kernel."""
+    """64 icospheres of uneven size (levels 4-7, 2,145,408 vertices) on the patch kernel."""
     scene = wl.config4()
     sim = wl.build_surface_gpu(scene)
     sim.set_kernel(fa.FP_KERNEL_AUTO, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
@@ -340,10 +343,10 @@ def bench_surfaces(args, np, fa, wl):
     kernel_ms = sim.time_steps_graph(args.steps, args.chunk)
     sim.synchronize()
     wall = time.perf_counter() - t0
-    # CSR algorithmic bytes per step: state 48 B/vertex, row pointers 2 x 4 B/vertex, 16 B per
-    # directed strut entry, 4 B per face corner entry, face vertex ids 12 B/face, and the face
-    # kernel's per-face results written once and read per corner (16 + 12 B written, 3 x (16 + 4) read)
-    bytes_step = 48 * N + 8 * N + 16 * 2 * S + 4 * 3 * F + 12 * F + 28 * F + 60 * F
+    # algorithmic bytes per step of a general mesh: the state read and written once (48 B/vertex),
+    # a row pointer (4 B/vertex), per directed strut entry the neighbour id and rest length (8 B),
+    # per face its 3 vertex ids and one constant (16 B)
+    bytes_step = 48 * N + 4 * N + 8 * 2 * S + 16 * F
     kernel_avg_s = kernel_ms * 1e-3 / args.steps
     achieved = bytes_step / kernel_avg_s / 1e9
     cpu = None
@@ -390,7 +393,7 @@ def bench_surfaces(args, np, fa, wl):
                                    f"the largest mesh's diameter {ext:.3f} m",
                "mlp_kernel_avg_us": mlp_ms * 1e3 / max(1, launches)}
     line = {
-        "metric": "sim-s/s, irregular meshes (64 icospheres, CSR); %HBM peak",
+        "metric": "sim-s/s, irregular meshes (64 icospheres); %HBM peak",
         "value": args.steps * dt / wall, "unit": "sim-s/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
@@ -398,8 +401,8 @@ def bench_surfaces(args, np, fa, wl):
                                f"{S} struts, {F} faces, ground plane, RK4 dt=1/600 s", "math": args.math},
         "vertex_steps_per_s": args.steps * N / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_avg_us": kernel_avg_s * 1e6,
-                     "bytes_per_launch": bytes_step},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_SURFACES.get(args.math),
+                     "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": bytes_step},
         "cpu_baseline": cpu,
         "surrogate": sur,
     }
