@@ -32,6 +32,24 @@ FP_HD float rcp_rn(float x) {
   return 1.0f / x;
 }
 
+// sqrt(x), correctly rounded.  On gfx950 for x in [2^-96, FLT_MAX]: the hardware square root
+// (within 1 ulp) and the choice among it and its two neighbours by the sign of the exact
+// residuals x - s' s (the compiler's own correctly rounded sequence without its rescaling of tiny
+// inputs and its zero/infinity fix-up, which that range does not need); checked on every such
+// input against the full sequence (fp_selftest_exact_math).  Other inputs take the full sequence.
+FP_HD float sqrt_rn(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  if (x >= 0x1p-96f && x <= 3.40282347e38f) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float dn = __uint_as_float(__float_as_uint(s) - 1u), up = __uint_as_float(__float_as_uint(s) + 1u);
+    const float rdn = __builtin_fmaf(-dn, s, x), rup = __builtin_fmaf(-up, s, x);
+    const float t = rdn <= 0.0f ? dn : s;
+    return rup > 0.0f ? up : t;
+  }
+#endif
+  return __builtin_sqrtf(x);
+}
+
 // x / PI (PI = std::f32::consts::PI = 0x40490fdb), correctly rounded: for x >= 2^-100 the product
 // with RN(1/PI) corrected once by the exact remainder fma(-q0, PI, x) (the same method as div3_rn;
 // below 2^-104 the remainder is no longer exact), otherwise (tiny, zero, negative, NaN) the

@@ -25,8 +25,8 @@ struct StrutK {
   float k, c, L;
 };
 __device__ __forceinline__ StrutK strut_k(float dX, float dY, float kn, float cn) {
-  const float L = __builtin_sqrtf(dX * dX + dY * dY);
-  const float inv = 1.0f / L;
+  const float L = sqrt_rn(dX * dX + dY * dY);
+  const float inv = rcp_rn(L);
   return StrutK{inv * kn, inv * cn, L};
 }
 

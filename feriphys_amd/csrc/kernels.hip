@@ -279,8 +279,8 @@ __global__ void __launch_bounds__(kBlock) grid_kernel(StepArgs A, const ClothVie
   const float xc = X(c), yr = Y(r);
   auto strut = [&](int rr, int cc, float kn, float cn) {
     const float dX = xc - X(cc), dY = yr - Y(rr);
-    const float L = __builtin_sqrtf(dX * dX + dY * dY);
-    const float inv = 1.0f / L;  // Strut::new: NOMINAL_STRUT_LENGTH / length (springy_mesh.rs:83-84)
+    const float L = sqrt_rn(dX * dX + dY * dY);
+    const float inv = rcp_rn(L);  // Strut::new: NOMINAL_STRUT_LENGTH / length (springy_mesh.rs:83-84)
     const uint32_t j = id(rr, cc);
     strut_accumulate<M>(F, p, v, ld3(S.x, S.y, S.z, j), ld3(S.vx, S.vy, S.vz, j), inv * kn, inv * cn, L);
   };
@@ -365,12 +365,12 @@ hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blo
   return hipGetLastError();
 }
 
-// Every finite f32 x (grid-stride over all bit patterns): the exact kernels' division-free forms
+// Every finite f32 x (grid-stride over all bit patterns): the exact kernels' shortened forms
 // against the operations they stand for, bad[0] rcp_rn(x) vs 1 / x, bad[1] div_pi_rn(x) vs x / PI,
 // bad[2] div3_rn(x) vs x / 3, bad[3] the branch-free atan2 vs fdlibm's control flow at y = 1 and
-// y = -1.7 (NaN payloads aside).
+// y = -1.7, bad[4] sqrt_rn(x) vs the compiler's correctly rounded sqrt (NaN payloads aside).
 __global__ void __launch_bounds__(kBlock) selftest_kernel(unsigned long long* __restrict__ bad) {
-  unsigned long long b[4] = {0, 0, 0, 0};
+  unsigned long long b[5] = {0, 0, 0, 0, 0};
   auto bits = [](float f) { return __float_as_uint(f); };
   for (uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x; k < (1ull << 32); k += (uint64_t)gridDim.x * kBlock) {
     const float x = __uint_as_float((uint32_t)k);
@@ -383,8 +383,10 @@ __global__ void __launch_bounds__(kBlock) selftest_kernel(unsigned long long* __
       const float a = fdlibm_atan2f(ys[j], x), r = fdlibm_atan2f_ref(ys[j], x);
       b[3] += bits(a) != bits(r) && !(__builtin_isnan(a) && __builtin_isnan(r));
     }
+    const float sq = sqrt_rn(x), sr = __builtin_sqrtf(x);
+    b[4] += bits(sq) != bits(sr) && !(__builtin_isnan(sq) && __builtin_isnan(sr));
   }
-  for (int q = 0; q < 4; ++q)
+  for (int q = 0; q < 5; ++q)
     if (b[q]) atomicAdd(&bad[q], b[q]);
 }
 

@@ -111,8 +111,8 @@ constexpr int kTileCols[N_TILE_SHAPES] = {15, 31, 16};
 
 hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
                       bool exact, hipStream_t s);
-// Counts, over every finite f32, where the exact kernels' division-free forms differ from the
-// operations they replace (bad[0..3], added to; see selftest_kernel).
+// Counts, over every finite f32, where the exact kernels' shortened forms differ from the
+// operations they replace (bad[0..4], added to; see selftest_kernel).
 hipError_t launch_selftest(unsigned long long* bad, hipStream_t s);
 // The step (MODE_STEP) of general meshes one patch per workgroup, the neighbour states read from
 // the patch's LDS copy; the same results as launch_csr.

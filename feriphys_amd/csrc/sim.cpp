@@ -623,16 +623,16 @@ extern "C" {
 const char* fp_last_error(void) { return g_err.c_str(); }
 const char* fp_version(void) { return "feriphys_gpu 0.1 gfx950"; }
 
-int fp_selftest_exact_math(uint64_t mismatches[4]) {
+int fp_selftest_exact_math(uint64_t mismatches[5]) {
   if (!mismatches) return fail(FP_EINVAL, "mismatches is NULL");
   DBuf<unsigned long long> d;
-  HIP_TRY(d.alloc(4));
-  HIP_TRY(hipMemset(d.p, 0, 4 * sizeof(unsigned long long)));
+  HIP_TRY(d.alloc(5));
+  HIP_TRY(hipMemset(d.p, 0, 5 * sizeof(unsigned long long)));
   HIP_TRY(fp::launch_selftest(d.p, nullptr));
   HIP_TRY(hipDeviceSynchronize());
-  unsigned long long h[4];
+  unsigned long long h[5];
   HIP_TRY(hipMemcpy(h, d.p, sizeof(h), hipMemcpyDeviceToHost));
-  for (int q = 0; q < 4; ++q) mismatches[q] = h[q];
+  for (int q = 0; q < 5; ++q) mismatches[q] = h[q];
   return FP_OK;
 }
 void fp_config_default(fp_config* out) {

@@ -81,7 +81,7 @@ __device__ __forceinline__ void strut_accumulate(f3& F, f3 p_self, f3 v_self, f3
   f3 d = sub(p_nbr, p_self);
   // |d| - rest cancels to the strain, so |d| is always formed exactly (unfused dot, IEEE sqrt):
   // one ulp of |d| is k * ulp(L), already ~5e-5 of a typical strut force.
-  float len = __builtin_sqrtf(dot(d, d));
+  float len = sqrt_rn(dot(d, d));
   float inv = M::rcp(len);
   f3 u = f3{d.x * inv, d.y * inv, d.z * inv};
   float s = k * (len - rest);

@@ -47,7 +47,7 @@ inline f3 h_normalize(f3 v) { return smul(1.0f / h_magnitude(v), v); }
 // no fused multiply-add in the reference's arithmetic: the reference's f32 results bit for bit.
 struct ExactMath {
   static constexpr bool kExact = true;
-  __device__ static float sqrt_(float x) { return __builtin_sqrtf(x); }
+  __device__ static float sqrt_(float x) { return sqrt_rn(x); }
   __device__ static float rcp(float x) { return rcp_rn(x); }
   __device__ static float div(float a, float b) { return a / b; }
   __device__ static float atan2_(float y, float x) { return fdlibm_atan2f(y, x); }

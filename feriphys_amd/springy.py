@@ -392,6 +392,7 @@ def config_default() -> Config:
 def selftest_exact_math() -> dict:
     """Runs fp_selftest_exact_math on the current GPU: over every finite f32, how often the exact
     kernels' division-free forms differ from the operations they replace (all 0 expected)."""
-    out = (C.c_uint64 * 4)()
+    out = (C.c_uint64 * 5)()
     check(lib().fp_selftest_exact_math(out))
-    return {"rcp": int(out[0]), "div_pi": int(out[1]), "div3": int(out[2]), "atan2": int(out[3])}
+    return {"rcp": int(out[0]), "div_pi": int(out[1]), "div3": int(out[2]), "atan2": int(out[3]),
+            "sqrt": int(out[4])}

@@ -76,10 +76,11 @@ const char* fp_last_error(void);
 /* Library version string. */
 const char* fp_version(void);
 /* No counterpart in the reference.  Checks on the current GPU, over every finite f32 x, that the
- * exact kernels' division-free forms equal the operations they stand for: mismatches[0] the
+ * exact kernels' shortened forms equal the operations they stand for: mismatches[0] the
  * reciprocal vs 1 / x, [1] x / PI, [2] x / 3, [3] the branch-free atan2 vs fdlibm's control flow
- * at y = 1 and y = -1.7.  All four are 0 on a GPU whose results the exact kernels reproduce. */
-int fp_selftest_exact_math(uint64_t mismatches[4]);
+ * at y = 1 and y = -1.7, [4] the square root vs the compiler's correctly rounded sequence.  All
+ * five are 0 on a GPU whose results the exact kernels reproduce. */
+int fp_selftest_exact_math(uint64_t mismatches[5]);
 /* Config::default() (springy/config.rs:20-33): RK4, dt = Duration::from_millis(1), gravity
  * (0,-10,0), no wind, lift 1, drag 1, restitution 0.95, friction 0.3. */
 void fp_config_default(fp_config* out);
