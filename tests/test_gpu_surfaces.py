@@ -77,3 +77,92 @@ def test_fan_with_many_faces_per_vertex_matches_oracle():
         a, b = g.get_points(m), o.mesh(m).points()
         assert np.isfinite(a).all()
         assert (bits(a) == bits(b)).all(), m
+
+
+def _orc_config(sc):
+    c = sc.config
+    return orc.Config(c.integration, c.dt, c.gravity, c.wind, c.lift_coefficient, c.drag_coefficient,
+                      c.coefficient_of_restitution, c.coefficient_of_friction)
+
+
+def test_icosphere_graph_replays_match_eager_steps():
+    """Captured step chunks on the patch kernel: every replay equals eager stepping, with readbacks
+    and synchronisations between the replays."""
+    sc = wl.config4(n_meshes=3, seed=11, levels=[3, 2, 4])
+    for m in sc.meshes:
+        m.positions[:, 1] -= m.positions[:, 1].min() - 0.005
+    eager, graph = wl.build_surface_gpu(sc), wl.build_surface_gpu(sc)
+    for rnd in range(3):
+        graph.step_graph(9, 9)
+        eager.step(9)
+        same = all((bits(graph.get_points(k)) == bits(eager.get_points(k))).all() for k in range(3))
+        announce(f"icosphere graph replay {rnd}: identical={same}")
+        assert same
+
+
+def test_icosphere_records_round_trip_and_update_points_match_oracle():
+    """A general mesh is stored in the library's own vertex order: records written and read back
+    come out unchanged, and SpringyMesh::update_points (pins kept, ground-plane collision) equals the
+    oracle bit for bit in the reference's order."""
+    sc = wl.config4(n_meshes=2, seed=13, levels=[3, 2])
+    g, o = wl.build_surface_gpu(sc), orc.build_surface_scene(sc)
+    rng = np.random.default_rng(5)
+    obs = [orc.Obstacle(np.asarray(p, np.float32), i) for p, i in sc.obstacles]
+    for k in range(2):
+        rec = o.mesh(k).points()
+        rec[:, 4:10] = rng.normal(scale=0.5, size=(rec.shape[0], 6)).astype(np.float32)
+        g.set_points(k, rec)
+        assert (bits(g.get_points(k)) == bits(rec)).all()
+        o.mesh(k).set_points(rec)
+        new = rec.copy()
+        new[:, 1:4] -= np.float32(1.2) * rec[:, 1:4].min(axis=0)[None, :] * np.array([0, 1, 0], np.float32)
+        new[:, 1:4] += rng.normal(scale=0.01, size=(rec.shape[0], 3)).astype(np.float32)
+        new[:, 4:7] = rng.normal(size=(rec.shape[0], 3)).astype(np.float32)
+        g.update_points(k, new)
+        o.mesh(k).update_points(new, obs, _orc_config(sc))
+        hits = int((bits(o.mesh(k).points()[:, 1:4]) != bits(new[:, 1:4])).any(axis=1).sum())
+        announce(f"mesh {k}: {rec.shape[0]} vertices, {hits} crossed the ground plane")
+        assert hits > 0
+        assert (bits(g.get_points(k)) == bits(o.mesh(k).points())).all()
+
+
+def test_icosphere_surrogate_slots_features_and_outputs():
+    """On a general mesh (stored in the library's own order) the surrogate's slots are the vertex's
+    struts in strut order, in the reference's vertex indices; its features and the network's outputs
+    match the network oracle's."""
+    import mlp_oracle as mo
+
+    sc = wl.config4(n_meshes=2, seed=17, levels=[2, 3])
+    sim, o = wl.build_surface_gpu(sc), orc.build_surface_scene(sc)
+    net = fa.Surrogate(seed=3)
+    sim.set_surrogate(net)
+    H = float(np.float32(np.float32(sc.config.dt) * np.float32(net.config.ratio)))
+    p = net.get_params().astype(np.float64)
+    rng = np.random.default_rng(9)
+    for m in range(2):
+        rec = sim.get_points(m)
+        rec[:, 4:7] = rng.normal(scale=0.3, size=(rec.shape[0], 3)).astype(np.float32)
+        sim.set_points(m, rec)
+        st = o.mesh(m).struts()
+        n = rec.shape[0]
+        exp_nbr = -np.ones((n, 12), np.int64)
+        exp_rest = np.zeros((n, 12), np.float32)
+        used = np.zeros(n, np.int64)
+        for (a, b), L in zip(st["v01"].astype(np.int64), st["kcl"][:, 2]):
+            if not L > 0:
+                continue
+            for s, t in ((a, b), (b, a)):
+                if used[s] < 12:
+                    exp_nbr[s, used[s]], exp_rest[s, used[s]] = t, L
+                    used[s] += 1
+        nbr, rest = sim.surrogate_slots(m)
+        assert (nbr == exp_nbr).all()
+        assert (bits(rest[exp_nbr >= 0]) == bits(exp_rest[exp_nbr >= 0])).all()
+        f = sim.surrogate_features(m)
+        fo = mo.features(rec[:, 1:4], rec[:, 4:7], nbr, rest, H, net.config.strain_scale)
+        np.testing.assert_allclose(f, fo, rtol=2e-5, atol=2e-6)
+        yo, _ = mo.forward(p, fo, emulate=True)
+        got = sim.surrogate_accel(m)
+        err = np.abs(got - yo * net.config.out_scale).max() / np.abs(yo * net.config.out_scale).max()
+        announce(f"surrogate on icosphere {m}: {n} vertices, output error {err:.2e} of the largest")
+        assert err < 1e-2
