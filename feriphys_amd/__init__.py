@@ -8,12 +8,14 @@ from ._lib import (FP_EULER, FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_K
                    FP_PLANE_XY, FP_PLANE_XZ, FP_RK4, FpError, lib)
 from .springy import (Cloth, CollidableMesh, Config, Integration, Simulation, SpringyMesh, config_default,
                       selftest_exact_math)
+from .rigid import RigidBody, RigidConfig, RigidSimulation, rigid_config_default
 from .surrogate import Surrogate, SurrogateConfig, SurrogateStats
 
 __all__ = [
     "Cloth", "CollidableMesh", "Config", "Integration", "Simulation", "SpringyMesh", "config_default", "FpError",
     "selftest_exact_math",
-    "Surrogate", "SurrogateConfig", "SurrogateStats",
+    "Surrogate", "SurrogateConfig", "SurrogateStats", "RigidBody", "RigidConfig", "RigidSimulation",
+    "rigid_config_default",
     "lib", "FP_EULER", "FP_RK4", "FP_PLANE_XY", "FP_PLANE_XZ", "FP_KERNEL_AUTO", "FP_KERNEL_CSR", "FP_KERNEL_GRID",
     "FP_KERNEL_GRID_SIMPLE",
     "FP_MATH_EXACT", "FP_MATH_FAST",

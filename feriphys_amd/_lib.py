@@ -50,6 +50,12 @@ class FpSurrogateStats(C.Structure):
                 ("plain_only", C.c_uint64), ("max_strain_residual", C.c_float), ("max_energy_residual", C.c_float)]
 
 
+class FpRigidConfig(C.Structure):
+    _fields_ = [("integration", C.c_int32), ("dt", C.c_float), ("coefficient_of_restitution", C.c_float),
+                ("gravity", C.c_float * 3), ("torque", C.c_float * 3)]
+
+
+FP_RIGID_RECORD = 29
 FP_SURROGATE_PARAMS = 42755
 FP_SURROGATE_FEATURES = 72
 
@@ -58,6 +64,21 @@ SIGNATURES = {
     "fp_last_error": (C.c_char_p, []),
     "fp_version": (C.c_char_p, []),
     "fp_selftest_exact_math": (C.c_int, [C.POINTER(C.c_uint64)]),
+    "fp_rigid_config_default": (None, [C.POINTER(FpRigidConfig)]),
+    "fp_rigid_new": (C.c_int, [_FP, C.c_float, _PP]),
+    "fp_rigid_get_state": (C.c_int, [_P, _FP]),
+    "fp_rigid_apply_impulse": (C.c_int, [_P, _FP, _FP]),
+    "fp_rigid_free": (None, [_P]),
+    "fp_rigid_sim_new": (C.c_int, [_PP, _U64, _PP, _U64, _PP]),
+    "fp_rigid_sim_free": (None, [_P]),
+    "fp_rigid_sim_count": (C.c_int, [_P, _U64P]),
+    "fp_rigid_sim_set_config": (C.c_int, [_P, C.POINTER(FpRigidConfig)]),
+    "fp_rigid_sim_get_config": (C.c_int, [_P, C.POINTER(FpRigidConfig)]),
+    "fp_rigid_sim_step": (C.c_int, [_P, _U64, _FP]),
+    "fp_rigid_sim_time_steps": (C.c_int, [_P, _U64, _FP]),
+    "fp_rigid_sim_get_states": (C.c_int, [_P, _FP]),
+    "fp_rigid_sim_set_states": (C.c_int, [_P, _FP, _U64]),
+    "fp_rigid_sim_apply_impulse": (C.c_int, [_P, _U64, _FP, _FP]),
     "fp_config_default": (None, [C.POINTER(FpConfig)]),
     "fp_mesh_new": (C.c_int, [_FP, _U64, _U64P, _U64, C.c_float, C.c_float, C.c_float, C.POINTER(FpTorsionConfig),
                               C.POINTER(FpStrutOverride), _U64, _PP]),

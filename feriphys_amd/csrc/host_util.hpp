@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstdint>
 #include <string>
 
 #include "../../include/feriphys_gpu.h"
@@ -18,6 +20,22 @@ inline int fail(int code, const std::string& msg) {
   return code;
 }
 inline int fail(const fp::Status& s) { return fail(s.code, s.msg); }
+
+// Duration::from_secs_f32(dt).as_secs_f32() (springy_mesh.rs:442; springy/simulation.rs:158).
+// The float -> Duration conversion is exact with round-half-to-even to whole nanoseconds: dt has
+// a 24-bit significand and 1e9 a 21-bit odd part, so dt * 1e9 is exact in x87 long double and
+// nearbyintl rounds it half-to-even.  Then secs as f32 + (nanos as f32) / 1e9.
+inline bool duration_roundtrip(float dt, float& out) {
+  if (!(dt >= 0.0f) || !std::isfinite(dt)) return false;
+  long double ns = (long double)dt * 1000000000.0L;
+  long double r = nearbyintl(ns);
+  if (r > 1.8e19L) return false;
+  uint64_t total = (uint64_t)r;
+  uint64_t secs = total / 1000000000ull;
+  uint32_t nanos = (uint32_t)(total % 1000000000ull);
+  out = (float)secs + (float)nanos / 1000000000.0f;
+  return true;
+}
 
 #define HIP_TRY(expr)                                                                    \
   do {                                                                                   \

@@ -1,0 +1,135 @@
+// Obstacle faces and broad-phase grids on the device (obstacles.hpp).
+#include "obstacles.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fph {
+
+namespace {
+
+// Obstacle faces and their broad-phase grids (see ObsView).  Faces whose dropped normal component
+// is at least kTau are binned in 3D; those between kThin and kTau in a 2D grid of their dropped
+// axis; the rest are tested for every vertex.  Boxes are grown by kMargin, far above the rounding
+// of the crossing point and of the containment test at the coordinates these scenes use.
+constexpr float kMargin = 1.0e-2f;
+constexpr float kTau = 0.05f;
+constexpr float kThin = 1.0e-3f;
+
+int build_grid(ObstacleSet& s, int g, const std::vector<fp::ObsFace>& faces, const std::vector<uint32_t>& ids,
+               int collapse) {
+  fp::GridView& G = s.view.grid[g];
+  G = fp::GridView{};
+  if (ids.empty()) return FP_OK;
+  auto box = [&](const fp::ObsFace& q, int k, float& l, float& h) {
+    l = std::min(q.v0[k], std::min(q.v1[k], q.v2[k])) - kMargin;
+    h = std::max(q.v0[k], std::max(q.v1[k], q.v2[k])) + kMargin;
+  };
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double ext_sum = 0.0;
+  int n_axes = 0;
+  for (uint32_t f : ids)
+    for (int k = 0; k < 3; ++k) {
+      if (k == collapse) continue;
+      float l, h;
+      box(faces[f], k, l, h);
+      lo[k] = std::min(lo[k], l);
+      hi[k] = std::max(hi[k], h);
+      ext_sum += (double)(h - l);
+    }
+  for (int k = 0; k < 3; ++k) n_axes += k != collapse;
+  const double mean_ext = ext_sum / ((double)n_axes * (double)ids.size());
+  for (int k = 0; k < 3; ++k) {
+    if (k == collapse) {
+      G.lo[k] = 0.0f;
+      G.inv_cell[k] = 0.0f;
+      G.dims[k] = 1;
+      continue;
+    }
+    const double ext = std::max((double)(hi[k] - lo[k]), 1e-6);
+    const double cell = std::max(mean_ext, ext / 128.0);
+    G.dims[k] = (int)std::min(128.0, std::max(1.0, std::ceil(ext / cell)));
+    G.inv_cell[k] = (float)((double)G.dims[k] / ext);
+    G.lo[k] = lo[k];
+  }
+  const uint64_t n_cells = (uint64_t)G.dims[0] * G.dims[1] * G.dims[2];
+  // the kernel's mapping from a coordinate to a cell, with the same f32 operations
+  auto range = [&](const fp::ObsFace& q, int k, int& c0, int& c1) {
+    float l, h;
+    box(q, k, l, h);
+    const float fl = (l - G.lo[k]) * G.inv_cell[k];
+    const float fh = (h - G.lo[k]) * G.inv_cell[k];
+    c0 = fl < 0.0f ? 0 : std::min((int)fl, G.dims[k] - 1);
+    c1 = fh >= (float)G.dims[k] ? G.dims[k] - 1 : std::max(0, (int)fh);
+  };
+  std::vector<uint32_t> start(n_cells + 1, 0);
+  auto visit = [&](auto&& fn) {
+    for (uint32_t f : ids) {  // ascending face ids -> ascending cell lists
+      int r[3][2];
+      for (int k = 0; k < 3; ++k) range(faces[f], k, r[k][0], r[k][1]);
+      for (int z = r[2][0]; z <= r[2][1]; ++z)
+        for (int y = r[1][0]; y <= r[1][1]; ++y)
+          for (int x = r[0][0]; x <= r[0][1]; ++x) fn(((uint64_t)z * G.dims[1] + y) * G.dims[0] + x, f);
+    }
+  };
+  visit([&](uint64_t c, uint32_t) { start[c + 1]++; });
+  for (uint64_t c = 0; c < n_cells; ++c) start[c + 1] += start[c];
+  std::vector<uint32_t> fill(start.begin(), start.end() - 1), lists(start[n_cells]);
+  visit([&](uint64_t c, uint32_t f) { lists[fill[c]++] = f; });
+  HIP_TRY(s.cell_start[g].upload(start.data(), start.size()));
+  HIP_TRY(s.cell_faces[g].upload(lists.data(), lists.size()));
+  G.cell_start = s.cell_start[g].p;
+  G.cell_faces = s.cell_faces[g].p;
+  return FP_OK;
+}
+
+}  // namespace
+
+int upload_obstacles(ObstacleSet& s, const std::vector<fp::HostObstacle>& obstacles) {
+  std::vector<fp::ObsFace> faces;
+  for (auto& o : obstacles) {
+    s.n_vertices += o.vertices.size() / 3;
+    for (uint64_t t = 0; t < o.n_faces(); ++t) {
+      const float* q = &o.faces[9 * t];
+      fp::ObsFace f{};
+      for (int k = 0; k < 3; ++k) { f.v0[k] = q[k]; f.v1[k] = q[3 + k]; f.v2[k] = q[6 + k]; }
+      fp::f3 a{q[0], q[1], q[2]}, b{q[3], q[4], q[5]}, c{q[6], q[7], q[8]};
+      fp::f3 n = fp::h_normalize(fp::cross(fp::sub(b, a), fp::sub(c, a)));  // Face::normal (collidable_mesh.rs:41-43)
+      f.n[0] = n.x; f.n[1] = n.y; f.n[2] = n.z;
+      // collidable_mesh.rs:133-154: signed comparisons, x first, then y, else z
+      f.axis = (n.x >= n.y && n.x >= n.z) ? 0 : ((n.y >= n.x && n.y >= n.z) ? 1 : 2);
+      faces.push_back(f);
+    }
+  }
+  s.n_faces = faces.size();
+  HIP_TRY(s.faces.upload(faces.data(), faces.size()));
+  fp::ObsView& O = s.view;
+  O = fp::ObsView{};
+  O.faces = s.faces.p;
+  O.n_faces = (uint32_t)faces.size();
+  O.margin = kMargin;
+  if (faces.size() <= 8) return FP_OK;  // scan every face in order
+
+  std::vector<uint32_t> always, ids[4];
+  for (uint32_t f = 0; f < faces.size(); ++f) {
+    const fp::ObsFace& q = faces[f];
+    const float an = std::fabs(q.n[q.axis]);
+    bool finite = true;
+    for (int k = 0; k < 3; ++k)
+      finite = finite && std::isfinite(q.v0[k]) && std::isfinite(q.v1[k]) && std::isfinite(q.v2[k]);
+    if (!finite || !(an >= kThin)) always.push_back(f);
+    else if (an >= kTau) ids[0].push_back(f);
+    else ids[1 + q.axis].push_back(f);
+  }
+  O.use_grid = 1;
+  s.n_always = always.size();
+  for (int g = 0; g < 4; ++g) s.n_binned[g] = ids[g].size();
+  HIP_TRY(s.always.upload(always.data(), always.size()));
+  O.always = s.always.p;
+  O.n_always = (uint32_t)always.size();
+  for (int g = 0; g < 4; ++g)
+    if (int e = build_grid(s, g, faces, ids[g], g == 0 ? -1 : g - 1)) return e;
+  return FP_OK;
+}
+
+}  // namespace fph

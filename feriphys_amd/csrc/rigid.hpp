@@ -1,0 +1,34 @@
+// Batched rigid bodies (rigidbody/rigidbody.rs, rigidbody/simulation.rs): device record layout,
+// step constants and the step launcher.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace fp {
+
+// A body's record: the reference's 29-float Stateful vector (rigidbody.rs:62-93: position,
+// rotation v then s, linear momentum, angular momentum, mass, initial inverse inertia by columns,
+// accumulated force, accumulated torque), stored in 32-float (128-byte) strides.
+constexpr int kRigidRecord = 29;
+constexpr int kRigidStride = 32;
+
+// The vertices of the unit cube every RigidBody::new builds (graphics/forms.rs:91-146), in its order.
+constexpr float kCubeCorner[8][3] = {{-0.5f, -0.5f, 0.5f}, {0.5f, -0.5f, 0.5f},   {0.5f, 0.5f, 0.5f},
+                                     {-0.5f, 0.5f, 0.5f},  {-0.5f, -0.5f, -0.5f}, {0.5f, -0.5f, -0.5f},
+                                     {0.5f, 0.5f, -0.5f},  {-0.5f, 0.5f, -0.5f}};
+
+// rigidbody/config.rs:7-13 plus Duration(dt) (the collision test's timestep).
+struct RigidParams {
+  int32_t integration;  // FP_EULER / FP_RK4
+  float dt;
+  float dt_d;  // Duration::from_secs_f32(dt).as_secs_f32()
+  float restitution;
+  f3 gravity;  // added to the accumulated force each step
+  f3 torque;   // added to the accumulated torque each step
+};
+
+// One Simulation::step of n bodies: records in -> out (kRigidStride floats per body).
+hipError_t launch_rigid_step(const float* in, float* out, uint32_t n, const RigidParams& p, const ObsView& obs,
+                             hipStream_t s);
+
+}  // namespace fp

@@ -318,6 +318,57 @@ int fp_sim_surrogate_features(fp_sim* sim, uint64_t mesh, float* features);
 int fp_sim_surrogate_accel(fp_sim* sim, uint64_t mesh, float* accel);
 int fp_sim_surrogate_slots(fp_sim* sim, uint64_t mesh, int64_t* nbr, float* rest);
 
+/* ---- rigid bodies (rigidbody/rigidbody.rs, rigidbody/simulation.rs) ----------------------- */
+/* A body is the reference's unit cube (graphics/forms.rs:91-146) with a 29-float Stateful record
+ * (rigidbody.rs:62-93): position, rotation (v.x, v.y, v.z, s), linear momentum, angular momentum,
+ * mass, the initial inverse inertia by columns, accumulated force, accumulated torque.  A
+ * simulation steps every body on the GPU, one thread per body, bit for bit the reference's f32
+ * arithmetic. */
+typedef struct fp_rigid fp_rigid;         /* RigidBody (rigidbody.rs:201-206)                      */
+typedef struct fp_rigid_sim fp_rigid_sim; /* rigidbody::Simulation (rigidbody/simulation.rs:10-14) */
+#define FP_RIGID_RECORD 29
+typedef struct fp_rigid_config { /* rigidbody/config.rs:7-13 */
+  int32_t integration;           /* FP_EULER / FP_RK4 */
+  float dt;
+  float coefficient_of_restitution;
+  float gravity[3]; /* added to the accumulated force every step */
+  float torque[3];  /* added to the accumulated torque every step */
+} fp_rigid_config;
+/* Config::default() (rigidbody/config.rs:15-25): RK4, dt 1 ms, restitution 0.7, no gravity or torque. */
+void fp_rigid_config_default(fp_rigid_config* out);
+/* RigidBody::new (rigidbody.rs:213-251): a unit cube at `position`, at rest, inertia mass/6 on the
+ * diagonal (inverted as cgmath's Matrix3::invert does); FP_EINVAL when it is not invertible. */
+int fp_rigid_new(const float position[3], float mass, fp_rigid** out);
+/* get_state().as_state() (rigidbody.rs:253-255, 62-93): the 29-float record. */
+int fp_rigid_get_state(const fp_rigid* body, float* record);
+/* RigidBody::apply_impulse (rigidbody.rs:345-352): impulse at `position` relative to the centre. */
+int fp_rigid_apply_impulse(fp_rigid* body, const float impulse[3], const float position[3]);
+void fp_rigid_free(fp_rigid* body);
+/* Simulation::new (rigidbody/simulation.rs:17-24): takes ownership of the bodies and obstacles
+ * (their handles are freed by this call, on success and on failure); the config starts as
+ * fp_rigid_config_default. */
+int fp_rigid_sim_new(fp_rigid* const* bodies, uint64_t n_bodies, fp_obstacle* const* obstacles, uint64_t n_obstacles,
+                     fp_rigid_sim** out);
+void fp_rigid_sim_free(fp_rigid_sim* sim);
+int fp_rigid_sim_count(const fp_rigid_sim* sim, uint64_t* n_bodies);
+/* The reference sets its Config only from its GUI (rigidbody/simulation.rs:64-76); these set and
+ * read it directly. */
+int fp_rigid_sim_set_config(fp_rigid_sim* sim, const fp_rigid_config* config);
+int fp_rigid_sim_get_config(const fp_rigid_sim* sim, fp_rigid_config* config);
+/* Simulation::step (rigidbody/simulation.rs:26-52) n_steps times; *timestep (may be NULL) receives
+ * Duration::from_secs_f32(dt) as f32. */
+int fp_rigid_sim_step(fp_rigid_sim* sim, uint64_t n_steps, float* timestep);
+/* No counterpart in the reference: n_steps steps bracketed by HIP events on the simulation's
+ * stream, in milliseconds. */
+int fp_rigid_sim_time_steps(fp_rigid_sim* sim, uint64_t n_steps, float* ms);
+/* get_rigidbodies()[i].get_state() for every body: n_bodies x 29 floats. */
+int fp_rigid_sim_get_states(fp_rigid_sim* sim, float* records);
+/* No counterpart in the reference: restores every body's record (a checkpoint written by
+ * fp_rigid_sim_get_states); n must be the body count. */
+int fp_rigid_sim_set_states(fp_rigid_sim* sim, const float* records, uint64_t n);
+/* rigidbodies[body].apply_impulse (rigidbody/simulation.rs:73-75 applies the GUI's impulse to body 0). */
+int fp_rigid_sim_apply_impulse(fp_rigid_sim* sim, uint64_t body, const float impulse[3], const float position[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -5,6 +5,7 @@
 // (suffix _f32 / _f64); the f64 form takes double arrays.  Errors that the reference reports by
 // panicking (bad index, bad state width, non-manifold edge) are returned as non-zero codes with a
 // message in the caller's buffer.
+#include "rigid_oracle.hpp"
 #include "springy_oracle.hpp"
 
 #include <cstdio>
@@ -259,6 +260,41 @@ template <class R> struct Sim {
     try { for (uint64_t i = 0; i < n; ++i) static_cast<Sim<R>*>(h)->sim.step(); return 0; } catch (...) { return 1; } \
   }                                                                                                                  \
   void* orc_sim_mesh##SFX(void* h, uint64_t i) { return &static_cast<Sim<R>*>(h)->sim.meshes[i]; }                   \
+  /* rigid bodies (rigid_oracle.hpp): RigidBody::new per body, the obstacles copied */                               \
+  void* orc_rigid_sim_new##SFX(const R* pos3, const R* mass, size_t n, void** obs, size_t nobs) {                    \
+    try {                                                                                                            \
+      auto* s = new RigidSimulation<R>();                                                                            \
+      for (size_t i = 0; i < n; ++i) s->bodies.emplace_back(V3<R>{pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2]}, mass[i]); \
+      for (size_t i = 0; i < nobs; ++i) s->obstacles.push_back(*static_cast<CollidableMesh<R>*>(obs[i]));            \
+      return s;                                                                                                      \
+    } catch (...) { return nullptr; }                                                                                \
+  }                                                                                                                  \
+  void orc_rigid_sim_free##SFX(void* h) { delete static_cast<RigidSimulation<R>*>(h); }                             \
+  void orc_rigid_sim_set_config##SFX(void* h, int integration, R dt, R restitution, const R* g3, const R* t3) {      \
+    auto& c = static_cast<RigidSimulation<R>*>(h)->config;                                                           \
+    c.integration = integration == 0 ? Integration::Euler : Integration::Rk4;                                        \
+    c.dt = dt; c.coefficient_of_restitution = restitution;                                                           \
+    c.gravity = {g3[0], g3[1], g3[2]}; c.torque = {t3[0], t3[1], t3[2]};                                             \
+  }                                                                                                                  \
+  int orc_rigid_sim_step##SFX(void* h, uint64_t n) {                                                                 \
+    try { for (uint64_t i = 0; i < n; ++i) static_cast<RigidSimulation<R>*>(h)->step(); return 0; } catch (...) { return 1; } \
+  }                                                                                                                  \
+  void orc_rigid_sim_get_states##SFX(void* h, R* out) {                                                              \
+    auto* s = static_cast<RigidSimulation<R>*>(h);                                                                   \
+    for (size_t i = 0; i < s->bodies.size(); ++i) {                                                                  \
+      auto v = s->bodies[i].state.as_state();                                                                        \
+      for (size_t k = 0; k < 29; ++k) out[29 * i + k] = v[k];                                                        \
+    }                                                                                                                \
+  }                                                                                                                  \
+  void orc_rigid_sim_set_states##SFX(void* h, const R* in) {                                                         \
+    auto* s = static_cast<RigidSimulation<R>*>(h);                                                                   \
+    for (size_t i = 0; i < s->bodies.size(); ++i)                                                                    \
+      s->bodies[i].state = RigidState<R>::from_state_vector(std::vector<R>(in + 29 * i, in + 29 * i + 29));          \
+  }                                                                                                                  \
+  void orc_rigid_sim_apply_impulse##SFX(void* h, uint64_t body, const R* impulse3, const R* at3) {                    \
+    static_cast<RigidSimulation<R>*>(h)->bodies[body].state.apply_impulse({impulse3[0], impulse3[1], impulse3[2]},   \
+                                                                          {at3[0], at3[1], at3[2]});                 \
+  }                                                                                                                  \
   }
 
 ORC_INSTANTIATE(float, _f32)

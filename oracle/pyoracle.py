@@ -92,6 +92,13 @@ def _declare(L):
             "orc_sim_set_config": (None, [_P, CP]),
             "orc_sim_step": (C.c_int, [_P, C.c_uint64]),
             "orc_sim_mesh": (_P, [_P, C.c_uint64]),
+            "orc_rigid_sim_new": (_P, [RP, RP, C.c_size_t, C.POINTER(_P), C.c_size_t]),
+            "orc_rigid_sim_free": (None, [_P]),
+            "orc_rigid_sim_set_config": (None, [_P, C.c_int, ct, ct, RP, RP]),
+            "orc_rigid_sim_step": (C.c_int, [_P, C.c_uint64]),
+            "orc_rigid_sim_get_states": (None, [_P, RP]),
+            "orc_rigid_sim_set_states": (None, [_P, RP]),
+            "orc_rigid_sim_apply_impulse": (None, [_P, C.c_uint64, RP, RP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name + sfx)
@@ -469,3 +476,52 @@ def tuple_difference(trio, duo):
     if r < 0:
         raise AssertionError("tuple_difference: duo is not a subset of trio")
     return r
+
+
+class RigidSimulation:
+    """rigidbody::Simulation (rigidbody/simulation.rs:10-52) of unit cubes (RigidBody::new,
+    rigidbody.rs:213-251) among static obstacles; states are the 29-float Stateful records
+    (rigidbody.rs:62-93)."""
+
+    def __init__(self, positions, masses, obstacles=(), dtype=np.float32):
+        self.dtype = np.dtype(dtype).type
+        ct = _CT[self.dtype]
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=self.dtype).reshape(-1, 3))
+        m = np.ascontiguousarray(np.asarray(masses, dtype=self.dtype).reshape(-1))
+        self.n = pos.shape[0]
+        self._obstacles = list(obstacles)
+        oa = (_P * max(1, len(obstacles)))(*[o.h for o in obstacles])
+        self.h = _fn("orc_rigid_sim_new", self.dtype)(_ptr(pos, ct), _ptr(m, ct), self.n, oa, len(obstacles))
+        if not self.h:
+            raise ValueError("Uninvertable moment of inertia!")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _fn("orc_rigid_sim_free", self.dtype)(self.h)
+            self.h = None
+
+    def set_config(self, integration=1, dt=0.001, restitution=0.7, gravity=(0, 0, 0), torque=(0, 0, 0)):
+        ct = _CT[self.dtype]
+        g = np.asarray(gravity, self.dtype)
+        t = np.asarray(torque, self.dtype)
+        _fn("orc_rigid_sim_set_config", self.dtype)(self.h, int(integration), self.dtype(dt), self.dtype(restitution),
+                                                    _ptr(g, ct), _ptr(t, ct))
+
+    def step(self, n=1):
+        if _fn("orc_rigid_sim_step", self.dtype)(self.h, n):
+            raise RuntimeError("rigid-body step failed")
+
+    def states(self):
+        out = np.zeros((self.n, 29), self.dtype)
+        _fn("orc_rigid_sim_get_states", self.dtype)(self.h, _ptr(out, _CT[self.dtype]))
+        return out
+
+    def set_states(self, states):
+        st = np.ascontiguousarray(np.asarray(states, self.dtype).reshape(self.n, 29))
+        _fn("orc_rigid_sim_set_states", self.dtype)(self.h, _ptr(st, _CT[self.dtype]))
+
+    def apply_impulse(self, body, impulse, at):
+        ct = _CT[self.dtype]
+        i = np.asarray(impulse, self.dtype)
+        a = np.asarray(at, self.dtype)
+        _fn("orc_rigid_sim_apply_impulse", self.dtype)(self.h, int(body), _ptr(i, ct), _ptr(a, ct))
