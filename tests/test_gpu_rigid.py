@@ -3,9 +3,10 @@ cubes with random impulses tumbling onto a ground plane and an icosphere (the br
 path), RK4 and Euler, with gravity and torque; every 29-float record bit for bit.
 
 The reference's response (one impulse per penetrating vertex, each from the pre-step velocity)
-adds energy in resting contact when the restitution is above 0: in the oracle itself a few cubes
-of a 96-cube pile reach NaN within 600 steps at restitution 0.6.  The long runs therefore use
-restitution 0; the run at 0.6 compares every value, a NaN where the oracle has one."""
+adds energy in resting contact when the restitution is above 0: in the oracle itself a cube of
+the 96-cube pile below reaches NaN at step 371 at restitution 0.6 (after which the sign of a NaN,
+which x86 and gfx950 produce differently, steers the collision tests).  The long runs therefore
+use restitution 0; the run at 0.6 stops at step 360, with momenta already growing."""
 import numpy as np
 import pytest
 
@@ -57,7 +58,7 @@ def same_bits(a, b):
 
 
 @pytest.mark.parametrize("integration,steps,restitution", [(1, (100, 300, 600), 0.0), (0, (50, 200), 0.0),
-                                                           (1, (200, 600), 0.6)])
+                                                           (1, (200, 360), 0.6)])
 def test_tumbling_cubes_bit_exact(integration, steps, restitution):
     g, o = build_both(96, 3 + integration, integration, (0.0, -9.81, 0.0), (0.0, 0.0, 0.0), restitution)
     assert (bits(g.states()) == bits(o.states())).all()
