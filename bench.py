@@ -44,9 +44,10 @@ def main():
     ap.add_argument("--kernel", choices=["auto", "grid", "grid_simple", "csr"], default="auto")
     ap.add_argument("--tile", choices=["default", "15x15", "7x31", "16x16"], default="default")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
-    ap.add_argument("--workload", choices=["cloth", "batched", "surfaces"], default="cloth",
+    ap.add_argument("--workload", choices=["cloth", "batched", "surfaces", "rigid"], default="cloth",
                     help="cloth: one 1024x1024 cloth + sphere (default); batched: 4096 cloths of uneven sizes; "
-                         "surfaces: 64 perturbed icospheres (2.1M vertices) on a ground plane")
+                         "surfaces: 64 perturbed icospheres (2.1M vertices) on a ground plane; rigid: 1M unit "
+                         "cubes tumbling onto a ground plane")
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph (batched)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "feriphys_amd", "weights", "cloth_surrogate.npy"),
                     help="surrogate network for the cloth workload's surrogate leg ('' to skip it)")
@@ -67,6 +68,8 @@ def main():
         return bench_batched(args, np, fa, wl)
     if args.workload == "surfaces":
         return bench_surfaces(args, np, fa, wl)
+    if args.workload == "rigid":
+        return bench_rigid(args, np, fa, wl)
     scene = wl.config1(n=args.n, steps=args.steps)
     sim = wl.build_gpu(scene)
     kern = {"auto": fa.FP_KERNEL_AUTO, "grid": fa.FP_KERNEL_GRID, "grid_simple": fa.FP_KERNEL_GRID_SIMPLE,
@@ -405,6 +408,82 @@ def bench_surfaces(args, np, fa, wl):
                      "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": bytes_step},
         "cpu_baseline": cpu,
         "surrogate": sur,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def rigid_scene(np, n, seed=0):
+    """n unit cubes on a jittered lattice 1-40 m above a ground plane, masses 0.5-4 kg, random
+    impulses; restitution 0 (the reference's vertex impulses add energy in resting contact above
+    0), gravity 9.81 m/s^2 as a force, RK4 at dt = 1/600 s."""
+    rng = np.random.default_rng(seed)
+    side = int(np.ceil(np.sqrt(n)))
+    i = np.arange(n)
+    pos = np.stack([(i % side) * 2.0 - side, 1.0 + rng.uniform(0.0, 39.0, n), (i // side) * 2.0 - side], 1)
+    mass = rng.uniform(0.5, 4.0, n)
+    imp = rng.normal(scale=2.0, size=(n, 3))
+    at = rng.uniform(-0.5, 0.5, size=(n, 3))
+    half = float(side) + 2.0
+    floor = (np.array([[-half, 0, half], [half, 0, half], [half, 0, -half], [-half, 0, -half]], np.float32),
+             np.array([0, 1, 2, 0, 2, 3], np.uint64))
+    return pos.astype(np.float32), mass.astype(np.float32), imp.astype(np.float32), at.astype(np.float32), floor
+
+
+def bench_rigid(args, np, fa, wl):
+    """1,048,576 unit cubes (the reference's RigidBody) tumbling onto a ground plane, one GPU thread
+    per body.  A "step" is one rigidbody::Simulation::step of every body."""
+    n = 1 << 20
+    pos, mass, imp, at, floor = rigid_scene(np, n)
+    bodies = [fa.RigidBody(pos[i], float(mass[i])) for i in range(n)]
+    sim = fa.RigidSimulation(bodies, [fa.CollidableMesh(*floor)])
+    rec = sim.states()
+    rec[:, 7:10] = rec[:, 7:10] + imp                  # apply_impulse: p += J, L += r x J
+    rec[:, 10:13] = rec[:, 10:13] + np.cross(at, imp)
+    sim.set_states(rec)
+    dt = float(np.float32(1.0 / 600.0))
+    sim.config = fa.RigidConfig(fa.Integration.Rk4, dt, 0.0, (0.0, -9.81, 0.0), (0.0, 0.0, 0.0))
+    if args.warmup:
+        sim.step(args.warmup)
+        sim.set_states(rec)
+    t0 = time.perf_counter()
+    kernel_ms = sim.time_steps(args.steps)
+    wall = time.perf_counter() - t0
+    out = sim.states()
+    if not np.isfinite(out).all():
+        print("non-finite state after the timed steps", file=sys.stderr)
+        sys.exit(3)
+    kernel_avg_s = kernel_ms * 1e-3 / args.steps
+    bytes_step = 2 * 29 * 4 * n  # the 29-float record read and written once per body-step
+    achieved = bytes_step / kernel_avg_s / 1e9
+    cpu = None
+    if args.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as orc
+
+        m = 4096
+        o = orc.RigidSimulation(pos[:m], mass[:m], [orc.Obstacle(*floor)])
+        o.set_states(rec[:m])
+        o.set_config(integration=1, dt=dt, restitution=0.0, gravity=(0.0, -9.81, 0.0))
+        steps = 10 * args.cpu_steps
+        t = time.perf_counter()
+        o.step(steps)
+        cpu_wall = time.perf_counter() - t
+        bsps = steps * m / cpu_wall
+        cpu = {"value": bsps / n * dt, "unit": "sim-s/s", "cores": 1, "kind": "port",
+               "sample": f"{steps} steps of the first {m} bodies, scaled by body count to {n}",
+               "body_steps_per_s": bsps}
+    line = {
+        "metric": "sim-s/s, 1M rigid cubes; %HBM peak",
+        "value": args.steps * dt / wall, "unit": "sim-s/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"rigidbody::Simulation::step of {n} unit cubes (random impulses, restitution 0) "
+                               f"onto a ground plane, RK4 dt=1/600 s", "bodies": n},
+        "body_steps_per_s": args.steps * n / wall,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_avg_us": kernel_avg_s * 1e6,
+                     "bytes_per_launch": bytes_step},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

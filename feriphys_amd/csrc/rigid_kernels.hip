@@ -75,10 +75,14 @@ __global__ void __launch_bounds__(kBlock) rigid_step_kernel(const float* __restr
                                                              uint32_t n, RigidParams P, ObsView O) {
   const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
   if (b >= n) return;
-  float S[kRigidRecord], T[kRigidRecord], K[kRigidRecord], A[kRigidRecord];
-  const float* src = in + (size_t)kRigidStride * b;
+  float S[kRigidStride], T[kRigidStride], K[kRigidRecord], A[kRigidRecord];
+  // the 128-byte record as eight 16-byte loads (one wavefront load instruction per 16 bytes)
+  const float4* src = reinterpret_cast<const float4*>(in + (size_t)kRigidStride * b);
 #pragma unroll
-  for (int k = 0; k < kRigidRecord; ++k) S[k] = src[k];
+  for (int q = 0; q < kRigidStride / 4; ++q) {
+    const float4 v = src[q];
+    S[4 * q] = v.x; S[4 * q + 1] = v.y; S[4 * q + 2] = v.z; S[4 * q + 3] = v.w;
+  }
   // accumulate_forces, accumulate_torques
   S[23] = S[23] + P.gravity.x; S[24] = S[24] + P.gravity.y; S[25] = S[25] + P.gravity.z;
   S[26] = S[26] + P.torque.x; S[27] = S[27] + P.torque.y; S[28] = S[28] + P.torque.z;
@@ -137,12 +141,12 @@ __global__ void __launch_bounds__(kBlock) rigid_step_kernel(const float* __restr
       T[10] = T[10] + dl.x; T[11] = T[11] + dl.y; T[12] = T[12] + dl.z;
     }
   }
-  // clear_forces, clear_torques
+  // clear_forces, clear_torques (and the record's padding)
 #pragma unroll
-  for (int k = 23; k < kRigidRecord; ++k) T[k] = 0.0f;
-  float* dst = out + (size_t)kRigidStride * b;
+  for (int k = 23; k < kRigidStride; ++k) T[k] = 0.0f;
+  float4* dst = reinterpret_cast<float4*>(out + (size_t)kRigidStride * b);
 #pragma unroll
-  for (int k = 0; k < kRigidRecord; ++k) dst[k] = T[k];
+  for (int q = 0; q < kRigidStride / 4; ++q) dst[q] = make_float4(T[4 * q], T[4 * q + 1], T[4 * q + 2], T[4 * q + 3]);
 }
 
 }  // namespace
