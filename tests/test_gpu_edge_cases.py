@@ -1,0 +1,68 @@
+"""Degenerate and boundary inputs on the GPU, each against the oracle: a simulation with no meshes,
+a general mesh with vertices that no triangle uses (patches of one vertex), pinned vertices on the
+patch kernel, a single triangle, and a rigid-body simulation with no bodies."""
+import numpy as np
+import pytest
+
+import feriphys_amd as fa
+import pyoracle as orc
+from gpu_common import announce, bits
+
+pytestmark = pytest.mark.gpu
+
+FLOOR = (np.array([[-5, 0, 5], [5, 0, 5], [5, 0, -5], [-5, 0, -5]], np.float32), np.array([0, 1, 2, 0, 2, 3], np.uint64))
+
+
+def cfg_pair(dt=1.0 / 600.0):
+    g = fa.Config(fa.Integration.Rk4, dt, (0.0, -9.81 * 0.01, 0.0), (1.0, 0.0, 0.5), 0.2, 0.5, 0.3, 0.2)
+    o = orc.Config(1, dt, (0.0, -9.81 * 0.01, 0.0), (1.0, 0.0, 0.5), 0.2, 0.5, 0.3, 0.2)
+    return g, o
+
+
+def test_simulation_without_meshes_steps():
+    sim = fa.Simulation([], [fa.CollidableMesh(*FLOOR)])
+    assert sim.step(5) == pytest.approx(0.001, rel=1e-6)
+
+
+def test_general_mesh_with_unused_vertices_and_pins():
+    """Two triangles sharing an edge plus three vertices no triangle uses; two pinned vertices."""
+    pos = np.array([[0, 0.3, 0], [0.1, 0.3, 0], [0, 0.3, 0.1], [0.1, 0.31, 0.1], [0.5, 0.5, 0.5], [0.6, 0.2, 0.0],
+                    [-0.4, 0.1, 0.3]], np.float32)
+    idx = np.array([0, 1, 2, 1, 3, 2], np.uint64)
+    gc, oc = cfg_pair()
+    gm = fa.SpringyMesh(pos, idx, 0.07, 20.0, 0.1)
+    om = orc.Mesh.new(pos, idx, 0.07, 20.0, 0.1)
+    for p in (1, 5):
+        gm.add_pin(p)
+        om.add_pin(p)
+    g = fa.Simulation([gm], [fa.CollidableMesh(*FLOOR)])
+    g.config = gc
+    o = orc.Simulation([om], [orc.Obstacle(*FLOOR)])
+    o.set_config(oc)
+    g.step(300)
+    o.step(300)
+    a, b = g.get_points(0), o.mesh(0).points()
+    announce(f"unused vertices: lowest y {a[:, 2].min():.4f}")
+    assert (bits(a) == bits(b)).all()
+    assert (a[[1, 5], 1:4] == pos[[1, 5]]).all()
+
+
+@pytest.mark.parametrize("kernel", [fa.FP_KERNEL_AUTO, fa.FP_KERNEL_CSR])
+def test_single_triangle(kernel):
+    pos = np.array([[0, 0.2, 0], [0.3, 0.25, 0], [0, 0.2, 0.3]], np.float32)
+    idx = np.array([0, 1, 2], np.uint64)
+    gc, oc = cfg_pair()
+    g = fa.Simulation([fa.SpringyMesh(pos, idx, 0.03, 10.0, 0.05)], [fa.CollidableMesh(*FLOOR)])
+    g.config = gc
+    g.set_kernel(kernel, fa.FP_MATH_EXACT)
+    o = orc.Simulation([orc.Mesh.new(pos, idx, 0.03, 10.0, 0.05)], [orc.Obstacle(*FLOOR)])
+    o.set_config(oc)
+    g.step(200)
+    o.step(200)
+    assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
+
+
+def test_rigid_simulation_without_bodies():
+    sim = fa.RigidSimulation([], [fa.CollidableMesh(*FLOOR)])
+    assert sim.step(3) == np.float32(0.001)
+    assert sim.states().shape == (0, 29)
