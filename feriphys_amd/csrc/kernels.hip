@@ -344,6 +344,19 @@ __global__ void __launch_bounds__(kBlock) update_points_kernel(StepArgs A, const
   A.fx[i] = Fn.x; A.fy[i] = Fn.y; A.fz[i] = Fn.z;
 }
 
+// Per mesh, the number of vertices whose position or velocity is not finite.
+__global__ void __launch_bounds__(kBlock) nonfinite_kernel(SoA S, const BlockRange* __restrict__ blocks,
+                                                            unsigned long long* __restrict__ counts) {
+  const BlockRange br = blocks[blockIdx.x];
+  const uint32_t i = br.begin + threadIdx.x;
+  bool bad = false;
+  if (i < br.end)
+    bad = !(__builtin_isfinite(S.x[i]) && __builtin_isfinite(S.y[i]) && __builtin_isfinite(S.z[i]) &&
+            __builtin_isfinite(S.vx[i]) && __builtin_isfinite(S.vy[i]) && __builtin_isfinite(S.vz[i]));
+  const int c = __syncthreads_count(bad);
+  if (threadIdx.x == 0 && c > 0) atomicAdd(&counts[br.mesh], (unsigned long long)c);
+}
+
 __global__ void fill_kernel(float* p, uint64_t n, float v) {
   for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
     p[k] = v;
@@ -429,6 +442,13 @@ hipError_t launch_update_points(const StepArgs& a, const float* nw_rec, uint32_t
                                 const BlockRange* blocks, uint32_t n_blocks, hipStream_t s) {
   if (n_blocks == 0) return hipSuccess;
   update_points_kernel<<<dim3(n_blocks), dim3(kBlock), 0, s>>>(a, nw_rec, rec_base, mass_out, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_nonfinite(const SoA& s_in, const BlockRange* blocks, uint32_t n_blocks, unsigned long long* counts,
+                            hipStream_t s) {
+  if (n_blocks == 0) return hipSuccess;
+  nonfinite_kernel<<<dim3(n_blocks), dim3(kBlock), 0, s>>>(s_in, blocks, counts);
   return hipGetLastError();
 }
 

@@ -111,6 +111,9 @@ constexpr int kTileCols[N_TILE_SHAPES] = {15, 31, 16};
 
 hipError_t launch_csr(const StepArgs& a, const CsrView& t, const BlockRange* blocks, uint32_t n_blocks, int mode,
                       bool exact, hipStream_t s);
+// Adds, per mesh (counts[mesh]), the vertices of `blocks` whose state has a NaN or infinity.
+hipError_t launch_nonfinite(const SoA& state, const BlockRange* blocks, uint32_t n_blocks, unsigned long long* counts,
+                            hipStream_t s);
 // Counts, over every finite f32, where the exact kernels' shortened forms differ from the
 // operations they replace (bad[0..4], added to; see selftest_kernel).
 hipError_t launch_selftest(unsigned long long* bad, hipStream_t s);

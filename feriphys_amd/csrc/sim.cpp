@@ -1118,6 +1118,22 @@ int fp_sim_mesh_counts(const fp_sim* s, uint64_t mesh, uint64_t* np, uint64_t* n
   return FP_OK;
 }
 
+int fp_sim_count_nonfinite(fp_sim* s, uint64_t* counts) {
+  if (!s) return fail(FP_EINVAL, "sim is NULL");
+  if (!counts && !s->meshes.empty()) return fail(FP_EINVAL, "counts is NULL");
+  const size_t M = s->meshes.size();
+  DBuf<unsigned long long> d;
+  HIP_TRY(d.alloc(M + 1));
+  HIP_TRY(hipMemsetAsync(d.p, 0, (M + 1) * sizeof(unsigned long long), s->stream));
+  fp::StepArgs a = s->args(false);
+  HIP_TRY(fp::launch_nonfinite(a.in, s->all_blocks.p, (uint32_t)s->h_blocks.size(), d.p, s->stream));
+  std::vector<unsigned long long> h(M + 1);
+  HIP_TRY(hipMemcpyAsync(h.data(), d.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  for (size_t m = 0; m < M; ++m) counts[m] = h[m];
+  return FP_OK;
+}
+
 int fp_sim_get_points(fp_sim* s, uint64_t mesh, float* rec) {
   if (int e = check_sim_mesh(s, mesh)) return e;
   if (!rec) return fail(FP_EINVAL, "records is NULL");

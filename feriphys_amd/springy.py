@@ -332,6 +332,13 @@ class Simulation(_Handle):
         check(lib().fp_sim_get_tile_shape(self.handle, mesh, C.byref(v)))
         return v.value
 
+    def count_nonfinite(self) -> np.ndarray:
+        """Per mesh, the vertices whose position or velocity is not finite (a diagnostic the
+        reference does not have; its steps, like these, let NaN propagate)."""
+        out = np.zeros(max(1, self.n_meshes), np.uint64)
+        check(lib().fp_sim_count_nonfinite(self.handle, _uptr(out)))
+        return out[:self.n_meshes]
+
     def time_steps(self, n_steps: int) -> float:
         """Runs n_steps bracketed by HIP events on the simulation's stream; returns milliseconds."""
         ms = C.c_float()

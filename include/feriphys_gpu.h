@@ -167,6 +167,10 @@ int fp_sim_get_timestep(const fp_sim* sim, float* out);
 int fp_sim_mesh_count(const fp_sim* sim, uint64_t* out);
 int fp_sim_mesh_counts(const fp_sim* sim, uint64_t mesh, uint64_t* n_points, uint64_t* n_struts,
                        uint64_t* n_faces, uint64_t* n_pins);
+/* No counterpart in the reference (it lets NaN propagate silently, as this library's steps do):
+ * per mesh, the number of vertices whose position or velocity holds a NaN or an infinity;
+ * counts has one entry per mesh. */
+int fp_sim_count_nonfinite(fp_sim* sim, uint64_t* counts);
 /* get_meshes()[mesh].get_points() (springy/simulation.rs:51-53, springy_mesh.rs:419-421). */
 int fp_sim_get_points(fp_sim* sim, uint64_t mesh, float* records);
 /* Checkpoint restore of one mesh's records (same layout; n must equal its point count). */

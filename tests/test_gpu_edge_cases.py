@@ -66,3 +66,23 @@ def test_rigid_simulation_without_bodies():
     sim = fa.RigidSimulation([], [fa.CollidableMesh(*FLOOR)])
     assert sim.step(3) == np.float32(0.001)
     assert sim.states().shape == (0, 29)
+
+
+def test_nonfinite_counts_per_mesh():
+    """A NaN planted in one vertex of the second cloth: counted there only, and (as in the
+    reference, which does not trap it) spreading through that cloth's struts as it steps."""
+    from feriphys_amd import workloads as wl
+
+    a = wl.config0(steps=1, seed=1, rows=8, cols=9, pins="edge")
+    b = wl.config0(steps=1, seed=2, rows=7, cols=6, pins="edge")
+    sc = wl.Scene(a.cloths + b.cloths, a.config, [], 1e-3, 1, 1, "two cloths")
+    sim = wl.build_gpu(sc)
+    assert list(sim.count_nonfinite()) == [0, 0]
+    rec = sim.get_points(1)
+    rec[20, 2] = np.nan
+    sim.set_points(1, rec)
+    assert list(sim.count_nonfinite()) == [0, 1]
+    sim.step(3)
+    c = sim.count_nonfinite()
+    announce(f"non-finite vertices after 3 steps: {list(c)}")
+    assert c[0] == 0 and c[1] > 1
