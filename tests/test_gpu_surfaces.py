@@ -166,3 +166,34 @@ def test_icosphere_surrogate_slots_features_and_outputs():
         err = np.abs(got - yo * net.config.out_scale).max() / np.abs(yo * net.config.out_scale).max()
         announce(f"surrogate on icosphere {m}: {n} vertices, output error {err:.2e} of the largest")
         assert err < 1e-2
+
+
+def test_kernel_switch_mid_run_matches_oracle():
+    """Patch kernel for 60 steps, then the CSR kernel for 60, then back: the block lists and face
+    shares are rebuilt on each switch and the trajectory stays the oracle's."""
+    sc = wl.config4(n_meshes=3, seed=21, levels=[3, 4, 2])
+    for m in sc.meshes:
+        m.positions[:, 1] -= m.positions[:, 1].min() - 0.01
+    g, o = wl.build_surface_gpu(sc), orc.build_surface_scene(sc)
+    for kernel in (fa.FP_KERNEL_AUTO, fa.FP_KERNEL_CSR, fa.FP_KERNEL_AUTO):
+        g.set_kernel(kernel, fa.FP_MATH_EXACT)
+        g.step(60)
+        o.step(60)
+        for k in range(3):
+            assert (bits(g.get_points(k)) == bits(o.mesh(k).points())).all(), (kernel, k)
+
+
+def test_config4_one_step_full_size():
+    """The 64-icosphere workload at full size (2,145,408 vertices): one step on the patch kernel
+    equals the oracle bit for bit."""
+    sc = wl.config4()
+    n = sum(m.positions.shape[0] for m in sc.meshes)
+    announce(f"config4 full size: {len(sc.meshes)} meshes, {n} vertices: building the library's meshes")
+    g = wl.build_surface_gpu(sc)
+    announce("building the oracle's meshes (about half a minute)")
+    o = orc.build_surface_scene(sc)
+    announce("one step on each side")
+    g.step(1)
+    o.step(1)
+    for k in range(len(sc.meshes)):
+        assert (bits(g.get_points(k)) == bits(o.mesh(k).points())).all(), k
