@@ -5,7 +5,8 @@ include/feriphys_gpu.h); this package is a thin ctypes mirror of the reference's
 """
 from ._lib import (FP_EULER, FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_KERNEL_GRID_SIMPLE, FP_MATH_EXACT,
                    FP_MATH_FAST,
-                   FP_PLANE_XY, FP_PLANE_XZ, FP_RK4, FpError, lib)
+                   FP_PLANE_XY, FP_PLANE_XZ, FP_RK4, FP_STEP_CSR, FP_STEP_GRID_SIMPLE, FP_STEP_PATCH, FP_STEP_TILED,
+                   FpError, lib)
 from .springy import (Cloth, CollidableMesh, Config, Integration, Simulation, SpringyMesh, config_default,
                       selftest_exact_math)
 from .rigid import RigidBody, RigidConfig, RigidSimulation, rigid_config_default
@@ -18,5 +19,5 @@ __all__ = [
     "rigid_config_default",
     "lib", "FP_EULER", "FP_RK4", "FP_PLANE_XY", "FP_PLANE_XZ", "FP_KERNEL_AUTO", "FP_KERNEL_CSR", "FP_KERNEL_GRID",
     "FP_KERNEL_GRID_SIMPLE",
-    "FP_MATH_EXACT", "FP_MATH_FAST",
+    "FP_MATH_EXACT", "FP_MATH_FAST", "FP_STEP_TILED", "FP_STEP_GRID_SIMPLE", "FP_STEP_PATCH", "FP_STEP_CSR",
 ]

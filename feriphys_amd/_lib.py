@@ -16,6 +16,7 @@ FP_PLANE_XY, FP_PLANE_XZ = 0, 1
 FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_KERNEL_GRID_SIMPLE = 0, 1, 2, 3
 FP_MATH_EXACT, FP_MATH_FAST = 0, 1
 FP_TILE_15x15, FP_TILE_7x31, FP_TILE_16x16 = 0, 1, 2
+FP_STEP_TILED, FP_STEP_GRID_SIMPLE, FP_STEP_PATCH, FP_STEP_CSR = 0, 1, 2, 3
 
 # Every symbol declared in include/feriphys_gpu.h, with its ctypes signature.
 _U64 = C.c_uint64
@@ -121,6 +122,7 @@ SIGNATURES = {
     "fp_sim_set_kernel": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "fp_sim_set_tile_shape": (C.c_int, [_P, _U64, C.c_int32]),
     "fp_sim_get_tile_shape": (C.c_int, [_P, _U64, C.POINTER(C.c_int32)]),
+    "fp_sim_get_mesh_kernel": (C.c_int, [_P, _U64, C.POINTER(C.c_int32)]),
     "fp_sim_autotune": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int32)]),
     "fp_sim_step_graph": (C.c_int, [_P, _U64, C.c_uint32]),
     "fp_sim_time_steps": (C.c_int, [_P, _U64, _FP]),

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 #include <string>
 
 #include "../../include/feriphys_gpu.h"
@@ -20,6 +21,14 @@ inline int fail(int code, const std::string& msg) {
   return code;
 }
 inline int fail(const fp::Status& s) { return fail(s.code, s.msg); }
+
+// Equal as bit patterns (so -0 differs from +0 and a NaN equals its own copy).
+inline bool same_bits(float a, float b) {
+  uint32_t x, y;
+  std::memcpy(&x, &a, 4);
+  std::memcpy(&y, &b, 4);
+  return x == y;
+}
 
 // Duration::from_secs_f32(dt).as_secs_f32() (springy_mesh.rs:442; springy/simulation.rs:158).
 // The float -> Duration conversion is exact with round-half-to-even to whole nanoseconds: dt has

@@ -104,20 +104,23 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
 
 // The step of one patch of a general mesh per workgroup (the operations of face_kernel and
 // csr_kernel, in the same order per vertex):
-//   phase 0  stage the states of the patch and its halo in LDS with one pass of loads;
+//   phase 0  stage the states of the patch and its halo in LDS with one pass of loads, and the
+//            mesh's table of nominal strut constants;
 //   phase 1  every face with a corner in the patch: drag + lift, and for each patch corner its
 //            share (angle / PI) * (drag + lift) into that vertex's LDS share slots (a face on the
 //            patch border is computed by each patch that holds one of its corners);
-//   phase 2  every patch vertex: struts from LDS in strut order, its face shares in face order,
-//            integrate, collide, pin, store.
+//   phase 2  every patch vertex: struts in strut order (8-byte entries, neighbour states from LDS),
+//            its face shares in face order, integrate, collide, pin, store.
 template <class M>
 __global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, const PatchRange* __restrict__ patches) {
   __shared__ float s_st[6][kPatchLocal];
   __shared__ float s_sh[3][kPatchFaces * kBlock];
+  __shared__ float2 s_cls[kPatchClasses];
   const PatchRange R = patches[blockIdx.x];
   const uint32_t np = R.end - R.begin, nl = np + R.n_halo;
   const SoA& S = A.in;
   const StepParams P = A.params[R.mesh];
+  if (threadIdx.x < kPatchClasses) s_cls[threadIdx.x] = T.p_cls[R.mesh * kPatchClasses + threadIdx.x];
   for (uint32_t e = threadIdx.x; e < nl; e += kBlock) {
     const uint32_t g = e < np ? R.begin + e : T.halo[R.halo_begin + (e - np)];
     s_st[0][e] = S.x[g]; s_st[1][e] = S.y[g]; s_st[2][e] = S.z[g];
@@ -147,23 +150,22 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, co
   const float m = P.uniform_mass ? P.mass : A.mass[i];
   f3 F = A.force_init ? f3{A.fx[i], A.fy[i], A.fz[i]} : f3{0.0f, 0.0f, 0.0f};
   F = add(F, P.gravity);  // apply_external_point_forces (:527-531)
-  // apply_strut_forces (:533-551), in strut order
+  // apply_strut_forces (:533-551), in strut order; each strut's stiffness and damping rebuilt as
+  // Strut::new makes them, (1.0 / len) * nominal (:83-84)
   {
-    const StrutEnt* ent = T.s_ent + T.s_first[i];
-    const uint16_t* loc = T.s_loc + T.s_first[i];
+    const PatchStrut* ent = T.p_ent + T.s_first[i];
     const uint32_t ns = T.s_cnt[i];
     for (uint32_t e = 0; e < ns; e += 4) {
-      StrutEnt se[4];
-      uint32_t l[4];
+      PatchStrut se[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const uint32_t k = kSlice * (e + j < ns ? e + j : e);
-        se[j] = ent[k];
-        l[j] = loc[k];
-      }
+      for (int j = 0; j < 4; ++j) se[j] = ent[kSlice * (e + j < ns ? e + j : e)];
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        if (e + j < ns) strut_accumulate<M>(F, p, v, Pl(l[j]), Vl(l[j]), se[j].k, se[j].c, se[j].len);
+        if (e + j < ns) {
+          const float2 kc = s_cls[se[j].cls];
+          const float il = rcp_rn(se[j].len);
+          strut_accumulate<M>(F, p, v, Pl(se[j].loc), Vl(se[j].loc), il * kc.x, il * kc.y, se[j].len);
+        }
     }
   }
   // apply_torsional_forces, when the step includes it (see csr_kernel)

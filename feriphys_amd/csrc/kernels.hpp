@@ -44,6 +44,15 @@ struct PatchFace {
 };
 constexpr uint16_t kNoShare = 0xffff;
 constexpr uint32_t kPatchFaces = 8;
+// A strut entry of a patch vertex (8 bytes): the rest length, the neighbour's index in the patch's
+// LDS copy and the strut's class, an index into its mesh's table of nominal (stiffness, damping)
+// pairs.  Strut::new scales both by NOMINAL_STRUT_LENGTH / length = 1.0 / len
+// (springy_mesh.rs:74-90), so the kernel recomputes them bit for bit as (1.0 / len) * nominal.
+struct alignas(8) PatchStrut {
+  float len;
+  uint16_t loc, cls;
+};
+constexpr uint32_t kPatchClasses = 16;  // classes per mesh; a mesh with more stays on the CSR kernel
 
 // Vertex-centric incidence lists of a packed set of meshes (global device vertex and face ids), in
 // a sliced layout: the vertices of one mesh are cut into slices of kSlice consecutive ids (one
@@ -61,9 +70,11 @@ struct CsrView {
   // per face entry: that corner's share of its face's drag + lift, (angle / PI) * (drag + lift),
   // written by the face kernel at f_slot
   float4* face_c;
-  // patch kernel: per strut entry the neighbour's index in its patch's LDS copy (the patch's own
-  // vertices first, then its halo), and the halo vertex ids of all patches
-  const uint16_t* s_loc;
+  // patch kernel: per strut entry (the same slots as s_ent) the compact entry, whose neighbour index
+  // is into its patch's LDS copy (the patch's own vertices first, then its halo); the class tables
+  // (mesh m's at m * kPatchClasses); the halo vertex ids of all patches
+  const PatchStrut* p_ent;
+  const float2* p_cls;
   const uint32_t* halo;
   const PatchFace* patch_faces;
   // torsional hinges in the step (nullptr: none): per vertex its (hinge << 2 | role) entries in

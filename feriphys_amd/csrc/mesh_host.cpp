@@ -46,6 +46,8 @@ inline void push_strut(HostMesh& m, uint32_t a, uint32_t b, float k_nom, float c
   m.s_v1.push_back(b);
   m.s_k.push_back(1.0f / len * k_nom);
   m.s_c.push_back(1.0f / len * c_nom);
+  m.s_kn.push_back(k_nom);
+  m.s_cn.push_back(c_nom);
   m.s_len.push_back(len);
   m.s_f0.push_back(-1);
   m.s_f1.push_back(-1);

@@ -30,6 +30,7 @@ struct HostMesh {
   // struts in reference index order
   std::vector<uint32_t> s_v0, s_v1;
   std::vector<float> s_k, s_c, s_len;
+  std::vector<float> s_kn, s_cn;  // the nominal constants that s_k, s_c were scaled from
   std::vector<int64_t> s_f0, s_f1;  // -1 = None
   std::vector<uint8_t> t_has;       // torsional spring present (springy_mesh.rs:348-393)
   std::vector<float> t_k, t_c, t_rest;

@@ -40,6 +40,7 @@ using fph::fail;
 using fph::g_err;
 
 using fph::duration_roundtrip;
+using fph::same_bits;
 
 fp_config default_config() {
   fp_config c;
@@ -74,7 +75,10 @@ struct MeshSlot {
   // device-local starts of the mesh's vertex blocks (one workgroup each), ending with n: patches of
   // at most kBlock connected vertices for general meshes, kBlock strides otherwise
   std::vector<uint32_t> bstart;
-  bool patch_ok = false;  // every patch's vertices and halo fit the patch kernel's LDS
+  // every patch's vertices and halo fit the patch kernel's LDS, no vertex has more than kPatchFaces
+  // faces, and the struts' nominal constants form at most kPatchClasses pairs
+  bool patch_ok = false;
+  bool on_patch = false;  // stepped by the patch kernel under the current selection (block lists)
   uint64_t dev(uint64_t i) const { return vbase + (pos.empty() ? i : pos[i]); }
   uint64_t ref(uint64_t d) const { return perm.empty() ? d : perm[d]; }
 };
@@ -176,10 +180,12 @@ struct fp_sim {
   DBuf<fp::StrutEnt> s_ent;
   DBuf<float4> face_c;
   // patch kernel: per block of a general mesh its patch and halo (h_patches[b] for block b), the
-  // halo ids, per strut entry the neighbour's LDS index; the launch list of the meshes on it
+  // halo ids, per strut entry its compact entry, the meshes' class tables; the launch list of the
+  // meshes on it
   std::vector<fp::PatchRange> h_patches;
   DBuf<uint32_t> halo;
-  DBuf<uint16_t> s_loc;
+  DBuf<fp::PatchStrut> p_ent;
+  DBuf<float2> p_cls;
   DBuf<fp::PatchFace> patch_faces;
   DBuf<fp::PatchRange> patch_list;
   uint32_t n_patch_list = 0;
@@ -266,7 +272,7 @@ struct fp_sim {
 
   fp::CsrView csr(bool with_torsion = false) const {
     const bool t = (with_torsion || torsion_on) && n_hinges > 0;
-    return fp::CsrView{s_first.p, s_cnt.p, s_ent.p, f_first.p, f_cnt.p, f_v.p, f_slot.p, face_c.p, s_loc.p, halo.p,
+    return fp::CsrView{s_first.p, s_cnt.p, s_ent.p, f_first.p, f_cnt.p, f_v.p, f_slot.p, face_c.p, p_ent.p, p_cls.p, halo.p,
                        patch_faces.p,
                        t ? t_row.p : nullptr, t ? t_ent.p : nullptr, hinge_f.p};
   }
@@ -354,6 +360,7 @@ int build_block_lists(fp_sim* s) {
                 : s->kernel == FP_KERNEL_GRID_SIMPLE   ? MK_GRID_SIMPLE
                                                        : MK_GRID_TILED;
     ms.tile_begin = ms.tile_end = (uint32_t)all_tiles.size();
+    ms.on_patch = false;
     if (ms.kernel == MK_GRID_TILED) {
       std::vector<fp::TileRange> mt;
       cloth_tiles(mi, ms.host.cloth, ms.tile_shape, mt);
@@ -365,6 +372,7 @@ int build_block_lists(fp_sim* s) {
     // general meshes: the patch kernel where every patch fits it, unless FP_KERNEL_CSR asks for the
     // one-thread-per-vertex CSR kernel (with the face kernel before it)
     const bool on_patch = ms.kernel == MK_CSR && ms.patch_ok && s->kernel != FP_KERNEL_CSR;
+    ms.on_patch = on_patch;
     for (uint32_t b = ms.blk_begin; b < ms.blk_end; ++b) {
       if (on_patch) pl.push_back(s->h_patches[b]);
       else (ms.kernel == MK_GRID_SIMPLE ? g : c).push_back(s->h_blocks[b]);
@@ -771,12 +779,31 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     std::vector<fp::StrutEnt> sent(n_sent + 1, fp::StrutEnt{0, 1.0f, 0.0f, 0.0f});
     std::vector<uint32_t> fslot(3 * F), sfill(N, 0), ffill(N, 0);
     std::vector<uint32_t> frank(3 * F);  // per face corner (reference face order): the face's rank among its vertex's faces
-    for (auto& ms_ : s->meshes) {
+    // the patch kernel's strut classes: per mesh the distinct nominal (stiffness, damping) pairs in
+    // first-appearance order, each strut's constants checked to be (1.0 / len) * its pair's
+    std::vector<uint16_t> scls(n_sent + 1, 0);
+    std::vector<float2> cls_tab(s->meshes.size() * fp::kPatchClasses, float2{0.0f, 0.0f});
+    std::vector<uint8_t> cls_ok(s->meshes.size(), 1);
+    for (size_t mi = 0; mi < s->meshes.size(); ++mi) {
+      auto& ms_ = s->meshes[mi];
       const auto& h = ms_.host;
+      uint32_t n_cls = 0;
       for (uint64_t k = 0; k < h.n_struts(); ++k) {
         uint32_t a = (uint32_t)ms_.dev(h.s_v0[k]), b = (uint32_t)ms_.dev(h.s_v1[k]);
-        sent[sfirst[a] + fp::kSlice * sfill[a]++] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
-        sent[sfirst[b] + fp::kSlice * sfill[b]++] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
+        const uint32_t ea = sfirst[a] + fp::kSlice * sfill[a]++, eb = sfirst[b] + fp::kSlice * sfill[b]++;
+        sent[ea] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
+        sent[eb] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
+        if (!cls_ok[mi]) continue;
+        const float kn = h.s_kn[k], cn = h.s_cn[k], inv = 1.0f / h.s_len[k];
+        if (!same_bits(inv * kn, h.s_k[k]) || !same_bits(inv * cn, h.s_c[k])) { cls_ok[mi] = 0; continue; }
+        float2* tab = cls_tab.data() + mi * fp::kPatchClasses;
+        uint32_t c = 0;
+        while (c < n_cls && !(same_bits(tab[c].x, kn) && same_bits(tab[c].y, cn))) ++c;
+        if (c == n_cls) {
+          if (n_cls == fp::kPatchClasses) { cls_ok[mi] = 0; continue; }
+          tab[n_cls++] = float2{kn, cn};
+        }
+        scls[ea] = scls[eb] = (uint16_t)c;
       }
       // faces are stored (for the face kernel) in ascending order of their lowest device vertex id,
       // so that neighbouring threads gather neighbouring vertices and write neighbouring entries;
@@ -810,7 +837,7 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
       auto& ms_ = s->meshes[mi];
       const auto& h = ms_.host;
-      ms_.patch_ok = !ms_.perm.empty();
+      ms_.patch_ok = !ms_.perm.empty() && cls_ok[mi];
       std::vector<uint32_t> vf_row, vf;  // reference faces of each device-local vertex
       if (ms_.patch_ok) {
         vf_row.assign(h.n + 1, 0);
@@ -864,7 +891,12 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     }
     pfaces.push_back(fp::PatchFace{});
     HIP_TRY(s->patch_faces.upload(pfaces.data(), pfaces.size()));
-    HIP_TRY(s->s_loc.upload(sloc.data(), sloc.size()));
+    {
+      std::vector<fp::PatchStrut> pent(n_sent + 1);
+      for (uint64_t e = 0; e <= n_sent; ++e) pent[e] = fp::PatchStrut{sent[e].len, sloc[e], scls[e]};
+      HIP_TRY(s->p_ent.upload(pent.data(), pent.size()));
+      HIP_TRY(s->p_cls.upload(cls_tab.data(), cls_tab.size()));
+    }
     halo.push_back(0);
     HIP_TRY(s->halo.upload(halo.data(), halo.size()));
     HIP_TRY(s->s_first.upload(sfirst.data(), sfirst.size()));
@@ -1294,6 +1326,17 @@ int fp_sim_get_tile_shape(const fp_sim* s, uint64_t mesh, int32_t* shape) {
   if (int e = check_sim_mesh(s, mesh)) return e;
   if (!shape) return fail(FP_EINVAL, "shape is NULL");
   *shape = s->meshes[mesh].tile_shape;
+  return FP_OK;
+}
+
+int fp_sim_get_mesh_kernel(const fp_sim* s, uint64_t mesh, int32_t* kernel) {
+  if (int e = check_sim_mesh(s, mesh)) return e;
+  if (!kernel) return fail(FP_EINVAL, "kernel is NULL");
+  const MeshSlot& m = s->meshes[mesh];
+  *kernel = m.kernel == MK_GRID_TILED    ? FP_STEP_TILED
+            : m.kernel == MK_GRID_SIMPLE ? FP_STEP_GRID_SIMPLE
+            : m.on_patch                 ? FP_STEP_PATCH
+                                         : FP_STEP_CSR;
   return FP_OK;
 }
 

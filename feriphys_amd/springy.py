@@ -332,6 +332,13 @@ class Simulation(_Handle):
         check(lib().fp_sim_get_tile_shape(self.handle, mesh, C.byref(v)))
         return v.value
 
+    def get_mesh_kernel(self, mesh: int = 0) -> int:
+        """The kernel stepping `mesh` now: FP_STEP_TILED, FP_STEP_GRID_SIMPLE, FP_STEP_PATCH or
+        FP_STEP_CSR."""
+        v = C.c_int32()
+        check(lib().fp_sim_get_mesh_kernel(self.handle, mesh, C.byref(v)))
+        return v.value
+
     def count_nonfinite(self) -> np.ndarray:
         """Per mesh, the vertices whose position or velocity is not finite (a diagnostic the
         reference does not have; its steps, like these, let NaN propagate)."""

@@ -199,7 +199,8 @@ int fp_sim_set_torsion(fp_sim* sim, int32_t enabled);
 /* Kernel selection.  FP_KERNEL_AUTO uses the LDS-tiled grid kernel for meshes built by Cloth::new
  * and left unchanged, and for other meshes the patch kernel (one workgroup per connected patch of
  * up to 256 vertices, patch and halo staged in LDS, faces computed in the same pass) when every
- * patch and halo fits it (at most 512 vertices, at most 8 faces per vertex), the CSR kernel (one
+ * patch and halo fits it (at most 512 vertices, at most 8 faces per vertex, at most 16 strut
+ * classes; see fp_sim_get_mesh_kernel), the CSR kernel (one
  * thread per vertex, after a face kernel) otherwise.  FP_KERNEL_CSR puts every mesh on the CSR
  * kernel; FP_KERNEL_GRID_SIMPLE is the one-thread-per-vertex grid kernel without LDS (kept as a
  * cross-check).  All kernels return the same results.  EXACT kernels follow the reference's operation
@@ -220,6 +221,16 @@ int fp_sim_set_kernel(fp_sim* sim, int32_t kernel, int32_t math);
 #define FP_TILE_16x16 2
 int fp_sim_set_tile_shape(fp_sim* sim, uint64_t mesh, int32_t shape);
 int fp_sim_get_tile_shape(const fp_sim* sim, uint64_t mesh, int32_t* shape);
+/* The kernel that steps one mesh under the current selection (no counterpart in the reference,
+ * whose step has one code path): FP_STEP_TILED (LDS-tiled grid kernel), FP_STEP_GRID_SIMPLE,
+ * FP_STEP_PATCH or FP_STEP_CSR.  The patch kernel also needs the mesh's struts to use at most 16
+ * distinct nominal (stiffness, damping) pairs (SpringyMesh::new's default, each override, each
+ * add_strut call's values). */
+#define FP_STEP_TILED 0
+#define FP_STEP_GRID_SIMPLE 1
+#define FP_STEP_PATCH 2
+#define FP_STEP_CSR 3
+int fp_sim_get_mesh_kernel(const fp_sim* sim, uint64_t mesh, int32_t* kernel);
 /* Autotuner: meshes on the tiled grid kernel are grouped by size class (longer side < 16, < 32,
  * < 64, >= 64 vertices); for each class every tile shape is timed over `steps_per_trial` steps
  * (HIP events on the simulation's stream, state restored afterwards) and the fastest is kept.

@@ -197,3 +197,42 @@ def test_config4_one_step_full_size():
     o.step(1)
     for k in range(len(sc.meshes)):
         assert (bits(g.get_points(k)) == bits(o.mesh(k).points())).all(), k
+
+
+@pytest.mark.parametrize("n_over", [14, 15])
+def test_strut_classes_choose_the_kernel_and_match_oracle(n_over):
+    """The patch kernel's 8-byte strut entries rebuild each strut's stiffness and damping as
+    (1.0 / len) * nominal from its mesh's table of nominal pairs, at most 16.  An icosphere whose
+    edge overrides make 16 pairs (the default, the bending struts' and 14 overrides) runs on it; one
+    more pair moves the mesh to the CSR kernel.  Both equal the oracle bit for bit over 120 steps."""
+    sc = wl.config4(n_meshes=1, seed=23, levels=[3])
+    spec = sc.meshes[0]
+    spec.positions[:, 1] -= spec.positions[:, 1].min() - 0.01
+    edges, seen = [], set()
+    for t in spec.indices.reshape(-1, 3).astype(np.int64):
+        for a, b in ((t[0], t[1]), (t[1], t[2]), (t[2], t[0])):
+            key = (int(min(a, b)), int(max(a, b)))
+            if key not in seen:
+                seen.add(key)
+                edges.append(key)
+    ovr = {e: (spec.k * (1.0 + 0.05 * (j + 1)), spec.c * (1.0 + 0.03 * (j + 1))) for j, e in enumerate(edges[:n_over])}
+    m = fa.SpringyMesh(spec.positions, spec.indices, spec.mass, spec.k, spec.c, strut_overrides=ovr)
+    om = orc.Mesh.new(spec.positions, spec.indices, spec.mass, spec.k, spec.c,
+                      overrides=[(a, b, kk, cc) for (a, b), (kk, cc) in ovr.items()])
+    v01, _, faces = m.struts()
+    _, fv = m.faces()
+    for a, b in wl.bend_pairs(v01, faces, fv):
+        m.add_strut((int(a), int(b)), spec.bk, spec.bc)
+        om.add_strut(int(a), int(b), spec.bk, spec.bc)
+    g = fa.Simulation([m], [fa.CollidableMesh(p, i) for p, i in sc.obstacles])
+    o = orc.Simulation([om], [orc.Obstacle(np.asarray(p, np.float32), i) for p, i in sc.obstacles])
+    c = sc.config
+    g.config = fa.Config(fa.Integration(c.integration), c.dt, c.gravity, c.wind, c.lift_coefficient,
+                         c.drag_coefficient, c.coefficient_of_restitution, c.coefficient_of_friction)
+    o.set_config(_orc_config(sc))
+    kernel = g.get_mesh_kernel(0)
+    announce(f"icosphere with {n_over} edge overrides: kernel {kernel}")
+    assert kernel == (fa.FP_STEP_PATCH if n_over == 14 else fa.FP_STEP_CSR)
+    g.step(120)
+    o.step(120)
+    assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
