@@ -29,9 +29,9 @@ BYTES_PER_VERTEX_STEP = 48
 # 1024 x 1024 cloth, from rocprofv3 FETCH_SIZE (13,212 KiB x 2.000) and WRITE_SIZE (24,576 KiB x
 # 1.000), each scaled by a 256 MiB device copy's reading (profiles/grid_tiled_exact_cloth1024.md)
 MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13212 * 2.0 + 24576 * 1.0) * 1024}
-# the same for patch_kernel<ExactMath> on the 64-icosphere workload (FETCH_SIZE 171,620 KiB x 2.000,
-# WRITE_SIZE 50,297 KiB x 1.000; profiles/batched_and_surfaces.md)
-MEASURED_TRAFFIC_SURFACES = {"exact": (171620 * 2.0 + 50297 * 1.0) * 1024}
+# the same for patch_kernel<ExactMath> on the 64-icosphere workload (FETCH_SIZE 216,682 KiB x 2.000,
+# WRITE_SIZE 50,318 KiB x 1.000; profiles/batched_and_surfaces.md)
+MEASURED_TRAFFIC_SURFACES = {"exact": (216682 * 2.0 + 50318 * 1.0) * 1024}
 
 
 def main():

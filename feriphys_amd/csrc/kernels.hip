@@ -106,15 +106,18 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
 // csr_kernel, in the same order per vertex):
 //   phase 0  stage the states of the patch and its halo in LDS with one pass of loads, and the
 //            mesh's table of nominal strut constants;
-//   phase 1  every face with a corner in the patch: drag + lift, and for each patch corner its
-//            share (angle / PI) * (drag + lift) into that vertex's LDS share slots (a face on the
-//            patch border is computed by each patch that holds one of its corners);
+//   phase 1  drag + lift of every face with a corner in the patch into LDS (a face on the patch
+//            border is computed by each patch that holds one of its corners);
 //   phase 2  every patch vertex: struts in strut order (8-byte entries, neighbour states from LDS),
-//            its face shares in face order, integrate, collide, pin, store.
+//            then its faces in face order, each its own corner's weight (angle / PI) times the
+//            face's drag + lift; integrate, collide, pin, store.
+// LDS is sized per launch: 6 x nl_cap floats of state and 3 x nf_cap of face forces.
 template <class M>
-__global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, const PatchRange* __restrict__ patches) {
-  __shared__ float s_st[6][kPatchLocal];
-  __shared__ float s_sh[3][kPatchFaces * kBlock];
+__global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, const PatchRange* __restrict__ patches,
+                                                        uint32_t nl_cap, uint32_t nf_cap) {
+  extern __shared__ float s_dyn[];
+  float* const s_st = s_dyn;                // [6][nl_cap]
+  float* const s_ff = s_dyn + 6 * nl_cap;   // [3][nf_cap]
   __shared__ float2 s_cls[kPatchClasses];
   const PatchRange R = patches[blockIdx.x];
   const uint32_t np = R.end - R.begin, nl = np + R.n_halo;
@@ -123,25 +126,16 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, co
   if (threadIdx.x < kPatchClasses) s_cls[threadIdx.x] = T.p_cls[R.mesh * kPatchClasses + threadIdx.x];
   for (uint32_t e = threadIdx.x; e < nl; e += kBlock) {
     const uint32_t g = e < np ? R.begin + e : T.halo[R.halo_begin + (e - np)];
-    s_st[0][e] = S.x[g]; s_st[1][e] = S.y[g]; s_st[2][e] = S.z[g];
-    s_st[3][e] = S.vx[g]; s_st[4][e] = S.vy[g]; s_st[5][e] = S.vz[g];
+    s_st[e] = S.x[g]; s_st[nl_cap + e] = S.y[g]; s_st[2 * nl_cap + e] = S.z[g];
+    s_st[3 * nl_cap + e] = S.vx[g]; s_st[4 * nl_cap + e] = S.vy[g]; s_st[5 * nl_cap + e] = S.vz[g];
   }
   __syncthreads();
-  auto Pl = [&](uint32_t l) { return f3{s_st[0][l], s_st[1][l], s_st[2][l]}; };
-  auto Vl = [&](uint32_t l) { return f3{s_st[3][l], s_st[4][l], s_st[5][l]}; };
+  auto Pl = [&](uint32_t l) { return f3{s_st[l], s_st[nl_cap + l], s_st[2 * nl_cap + l]}; };
+  auto Vl = [&](uint32_t l) { return f3{s_st[3 * nl_cap + l], s_st[4 * nl_cap + l], s_st[5 * nl_cap + l]}; };
   for (uint32_t q = threadIdx.x; q < R.n_faces; q += kBlock) {
     const PatchFace pf = T.patch_faces[R.face_begin + q];
-    const f3 p0 = Pl(pf.v[0]), p1 = Pl(pf.v[1]), p2 = Pl(pf.v[2]);
-    const f3 sum = face_force<M>(p0, p1, p2, Vl(pf.v[0]), Vl(pf.v[1]), Vl(pf.v[2]), P);
-    const f3 c[3] = {p0, p1, p2};
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      if (pf.share[k] == kNoShare) continue;
-      const float w = corner_weight<M>(c[k], c[(k + 1) % 3], c[(k + 2) % 3]);
-      s_sh[0][pf.share[k]] = w * sum.x;
-      s_sh[1][pf.share[k]] = w * sum.y;
-      s_sh[2][pf.share[k]] = w * sum.z;
-    }
+    const f3 sum = face_force<M>(Pl(pf.v[0]), Pl(pf.v[1]), Pl(pf.v[2]), Vl(pf.v[0]), Vl(pf.v[1]), Vl(pf.v[2]), P);
+    s_ff[q] = sum.x; s_ff[nf_cap + q] = sum.y; s_ff[2 * nf_cap + q] = sum.z;
   }
   __syncthreads();
   if (threadIdx.x >= np) return;
@@ -181,9 +175,24 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, co
       F = add(F, ts);
     }
   }
-  // apply_face_forces (:629-654): this vertex's face shares, in face order
-  for (uint32_t j = 0, nf = T.f_cnt[i]; j < nf; ++j)
-    F = add(F, f3{s_sh[0][j * kBlock + t], s_sh[1][j * kBlock + t], s_sh[2][j * kBlock + t]});
+  // apply_face_forces (:629-654): in face order, this vertex's corner weight (vertex_angle / PI,
+  // :647-649) times the face's drag + lift (:650-652)
+  {
+    const PatchCorner* cor = T.p_corner + T.f_first[i];
+    const uint32_t nf = T.f_cnt[i];
+    for (uint32_t j = 0; j < nf; j += 4) {
+      PatchCorner ce[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) ce[k] = cor[kSlice * (j + k < nf ? j + k : j)];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (j + k < nf) {
+          const float w = corner_weight<M>(p, Pl(ce[k].next), Pl(ce[k].prev));
+          const uint32_t q = ce[k].q;
+          F = add(F, f3{w * s_ff[q], w * s_ff[nf_cap + q], w * s_ff[2 * nf_cap + q]});
+        }
+    }
+  }
   const bool pin = pinned(A.pins, i);
   if (pin) F = f3{0.0f, 0.0f, 0.0f};  // (:522-524)
   finish_step<M>(A, P, i, m, p, v, F, pin);
@@ -411,10 +420,12 @@ hipError_t launch_selftest(unsigned long long* bad, hipStream_t s) {
 }
 
 hipError_t launch_patches(const StepArgs& a, const CsrView& t, const PatchRange* patches, uint32_t n_patches,
-                          bool exact, hipStream_t s) {
+                          uint32_t nl_cap, uint32_t nf_cap, bool exact, hipStream_t s) {
   if (n_patches == 0) return hipSuccess;
-  if (exact) patch_kernel<ExactMath><<<dim3(n_patches), dim3(kBlock), 0, s>>>(a, t, patches);
-  else patch_kernel<FastMath><<<dim3(n_patches), dim3(kBlock), 0, s>>>(a, t, patches);
+  if (nl_cap == 0 || nl_cap > kPatchLocal || nf_cap > kPatchFaceCap) return hipErrorInvalidValue;
+  const size_t lds = sizeof(float) * (6 * (size_t)nl_cap + 3 * (size_t)nf_cap);
+  if (exact) patch_kernel<ExactMath><<<dim3(n_patches), dim3(kBlock), lds, s>>>(a, t, patches, nl_cap, nf_cap);
+  else patch_kernel<FastMath><<<dim3(n_patches), dim3(kBlock), lds, s>>>(a, t, patches, nl_cap, nf_cap);
   return hipGetLastError();
 }
 

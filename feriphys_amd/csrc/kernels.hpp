@@ -31,19 +31,22 @@ struct HingeEnt {
 // of at most kBlock vertices), its halo, halo[halo_begin .. halo_begin + n_halo) (every vertex
 // outside the patch that a patch vertex's struts or faces reach; patch plus halo is at most
 // kPatchLocal), and its faces, patch_faces[face_begin .. face_begin + n_faces) (every face with a
-// corner in the patch).
+// corner in the patch, at most kPatchFaceCap).
 struct PatchRange {
   uint32_t mesh, begin, end, halo_begin, n_halo, face_begin, n_faces, pad;
 };
 constexpr uint32_t kPatchLocal = 512;
-// A face of a patch: its corners' indices in the patch's LDS copy, and for each corner that is a
-// patch vertex t, the LDS slot of its share, j * kBlock + t for the face's rank j among t's faces
-// in face-index order (kNoShare: a halo corner).  A patch vertex has at most kPatchFaces faces.
-struct PatchFace {
-  uint16_t v[3], share[3];
+constexpr uint32_t kPatchFaceCap = 2048;
+// A face of a patch: its corners' indices in the patch's LDS copy.
+struct alignas(8) PatchFace {
+  uint16_t v[3], pad;
 };
-constexpr uint16_t kNoShare = 0xffff;
-constexpr uint32_t kPatchFaces = 8;
+// A face corner of a patch vertex (one per face entry of the vertex, in face-index order): the
+// face's index in its patch's face list and the LDS indices of the face's next and previous
+// corners, the operands of Face::vertex_angle for this corner.
+struct alignas(8) PatchCorner {
+  uint16_t q, next, prev, pad;
+};
 // A strut entry of a patch vertex (8 bytes): the rest length, the neighbour's index in the patch's
 // LDS copy and the strut's class, an index into its mesh's table of nominal (stiffness, damping)
 // pairs.  Strut::new scales both by NOMINAL_STRUT_LENGTH / length = 1.0 / len
@@ -77,6 +80,7 @@ struct CsrView {
   const float2* p_cls;
   const uint32_t* halo;
   const PatchFace* patch_faces;
+  const PatchCorner* p_corner;  // per face entry (the slots of face_c)
   // torsional hinges in the step (nullptr: none): per vertex its (hinge << 2 | role) entries in
   // strut order (plain CSR, N+1 row pointers), and the 4 forces per hinge written by the hinge kernel
   const uint32_t* t_row;
@@ -129,9 +133,10 @@ hipError_t launch_nonfinite(const SoA& state, const BlockRange* blocks, uint32_t
 // operations they replace (bad[0..4], added to; see selftest_kernel).
 hipError_t launch_selftest(unsigned long long* bad, hipStream_t s);
 // The step (MODE_STEP) of general meshes one patch per workgroup, the neighbour states read from
-// the patch's LDS copy; the same results as launch_csr.
+// the patch's LDS copy; the same results as launch_csr.  nl_cap and nf_cap bound every patch's
+// vertices plus halo (<= kPatchLocal) and faces (<= kPatchFaceCap): they size its LDS.
 hipError_t launch_patches(const StepArgs& a, const CsrView& t, const PatchRange* patches, uint32_t n_patches,
-                          bool exact, hipStream_t s);
+                          uint32_t nl_cap, uint32_t nf_cap, bool exact, hipStream_t s);
 // Drag + lift and corner weights of the faces in `face_blocks` (BlockRange over global face ids),
 // for the CSR kernel that runs after it in the same stream.
 hipError_t launch_faces(const StepArgs& a, const CsrView& t, const BlockRange* face_blocks, uint32_t n_blocks,

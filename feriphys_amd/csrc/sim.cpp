@@ -75,8 +75,8 @@ struct MeshSlot {
   // device-local starts of the mesh's vertex blocks (one workgroup each), ending with n: patches of
   // at most kBlock connected vertices for general meshes, kBlock strides otherwise
   std::vector<uint32_t> bstart;
-  // every patch's vertices and halo fit the patch kernel's LDS, no vertex has more than kPatchFaces
-  // faces, and the struts' nominal constants form at most kPatchClasses pairs
+  // every patch's vertices and halo fit the patch kernel's LDS (at most kPatchLocal, faces at most
+  // kPatchFaceCap), and the struts' nominal constants form at most kPatchClasses pairs
   bool patch_ok = false;
   bool on_patch = false;  // stepped by the patch kernel under the current selection (block lists)
   uint64_t dev(uint64_t i) const { return vbase + (pos.empty() ? i : pos[i]); }
@@ -187,8 +187,10 @@ struct fp_sim {
   DBuf<fp::PatchStrut> p_ent;
   DBuf<float2> p_cls;
   DBuf<fp::PatchFace> patch_faces;
+  DBuf<fp::PatchCorner> p_corner;
   DBuf<fp::PatchRange> patch_list;
   uint32_t n_patch_list = 0;
+  uint32_t patch_nl_cap = 1, patch_nf_cap = 0;  // largest vertices + halo, faces of a listed patch
   // face blocks of the meshes on the CSR kernel (BlockRange over global face ids), all meshes'
   // face blocks (for the split API, indexed per mesh by fblk_begin/fblk_end)
   DBuf<fp::BlockRange> csr_face_blocks, all_face_blocks;
@@ -273,7 +275,7 @@ struct fp_sim {
   fp::CsrView csr(bool with_torsion = false) const {
     const bool t = (with_torsion || torsion_on) && n_hinges > 0;
     return fp::CsrView{s_first.p, s_cnt.p, s_ent.p, f_first.p, f_cnt.p, f_v.p, f_slot.p, face_c.p, p_ent.p, p_cls.p, halo.p,
-                       patch_faces.p,
+                       patch_faces.p, p_corner.p,
                        t ? t_row.p : nullptr, t ? t_ent.p : nullptr, hinge_f.p};
   }
 };
@@ -388,6 +390,12 @@ int build_block_lists(fp_sim* s) {
   c = xcd_order(c);
   pl = xcd_order(pl);
   s->n_patch_list = (uint32_t)pl.size();
+  s->patch_nl_cap = 1;
+  s->patch_nf_cap = 0;
+  for (const auto& r : pl) {
+    s->patch_nl_cap = std::max(s->patch_nl_cap, r.end - r.begin + r.n_halo);
+    s->patch_nf_cap = std::max(s->patch_nf_cap, r.n_faces);
+  }
   HIP_TRY(s->patch_list.upload(pl.data(), pl.size()));
   cf = xcd_order(cf);
   sb = xcd_order(sb);
@@ -436,7 +444,8 @@ int run_steps(fp_sim* s, uint64_t n) {
     if (s->torsion_on)
       HIP_TRY(fp::launch_hinges(a, s->csr(), s->hinges.p, 0, s->n_hinges, exact, s->stream));
     HIP_TRY(fp::launch_csr(a, s->csr(), s->csr_blocks.p, s->n_csr_blocks, fp::MODE_STEP, exact, s->stream));
-    HIP_TRY(fp::launch_patches(a, s->csr(), s->patch_list.p, s->n_patch_list, exact, s->stream));
+    HIP_TRY(fp::launch_patches(a, s->csr(), s->patch_list.p, s->n_patch_list, s->patch_nl_cap, s->patch_nf_cap,
+                               exact, s->stream));
     s->cur ^= 1;
     s->forces_dirty = false;
   }
@@ -828,11 +837,12 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
         }
     }
     // patches of the general meshes (block b of every mesh is h_patches[b]): halo, faces, LDS
-    // indices.  A mesh with a patch whose vertices and halo exceed kPatchLocal, or with a vertex of
-    // more than kPatchFaces faces, stays on the CSR kernel.
+    // indices, each patch vertex's corners.  A mesh with a patch whose vertices and halo exceed
+    // kPatchLocal, or whose faces exceed kPatchFaceCap, stays on the CSR kernel.
     std::vector<uint16_t> sloc(n_sent + 1, 0);
     std::vector<uint32_t> halo;
     std::vector<fp::PatchFace> pfaces;
+    std::vector<fp::PatchCorner> pcorner(n_fent + 1, fp::PatchCorner{0, 0, 0, 0});
     std::vector<int32_t> mark(N, -1);
     for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
       auto& ms_ = s->meshes[mi];
@@ -846,8 +856,6 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
         vf.resize(h.f_v.size());
         std::vector<uint32_t> fill(vf_row.begin(), vf_row.end() - 1);
         for (uint64_t k = 0; k < h.f_v.size(); ++k) vf[fill[ms_.pos[h.f_v[k]]]++] = (uint32_t)(k / 3);
-        for (uint64_t i = ms_.vbase; i < ms_.vbase + h.n; ++i)
-          if (fcnt[i] > fp::kPatchFaces) ms_.patch_ok = false;
       }
       std::vector<uint8_t> fseen(ms_.patch_ok ? h.n_faces() : 0, 0);
       for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b) {
@@ -869,21 +877,28 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
           for (uint32_t i = b0; i < b1; ++i)
             for (uint32_t k = vf_row[i - ms_.vbase]; k < vf_row[i - ms_.vbase + 1]; ++k)
               if (!fseen[vf[k]]) { fseen[vf[k]] = 1; fl.push_back(vf[k]); }
-          for (uint32_t f : fl) {
+          for (uint32_t qi = 0; qi < (uint32_t)fl.size(); ++qi) {
+            const uint32_t f = fl[qi];
             fp::PatchFace pf{};
+            uint32_t g3[3];
             for (int c = 0; c < 3; ++c) {
-              const uint32_t g = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)f + c]);
-              pf.v[c] = (uint16_t)std::min<uint32_t>(local(g), fp::kPatchLocal - 1);
-              pf.share[c] = (g >= b0 && g < b1)
-                                ? (uint16_t)(frank[3 * (ms_.fbase + f) + c] * fp::kBlock + (g - b0))
-                                : fp::kNoShare;
+              g3[c] = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)f + c]);
+              pf.v[c] = (uint16_t)std::min<uint32_t>(local(g3[c]), fp::kPatchLocal - 1);
             }
+            // corner c of a patch vertex: Face::vertex_angle's operands are the next and the
+            // previous corner (springy_mesh.rs:163-172), in the entry of the face's rank among
+            // that vertex's faces
+            for (int c = 0; c < 3; ++c)
+              if (g3[c] >= b0 && g3[c] < b1)
+                pcorner[ffirst[g3[c]] + fp::kSlice * frank[3 * (ms_.fbase + f) + c]] =
+                    fp::PatchCorner{(uint16_t)std::min<uint32_t>(qi, fp::kPatchFaceCap - 1), pf.v[(c + 1) % 3],
+                                    pf.v[(c + 2) % 3], 0};
             pfaces.push_back(pf);
           }
           for (uint32_t f : fl) fseen[f] = 0;
           pr.n_halo = (uint32_t)(halo.size() - h0);
           pr.n_faces = (uint32_t)(pfaces.size() - q0);
-          if ((b1 - b0) + pr.n_halo > fp::kPatchLocal) ms_.patch_ok = false;
+          if ((b1 - b0) + pr.n_halo > fp::kPatchLocal || pr.n_faces > fp::kPatchFaceCap) ms_.patch_ok = false;
           for (size_t k = h0; k < halo.size(); ++k) mark[halo[k]] = -1;
         }
         s->h_patches.push_back(pr);
@@ -891,6 +906,7 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     }
     pfaces.push_back(fp::PatchFace{});
     HIP_TRY(s->patch_faces.upload(pfaces.data(), pfaces.size()));
+    HIP_TRY(s->p_corner.upload(pcorner.data(), pcorner.size()));
     {
       std::vector<fp::PatchStrut> pent(n_sent + 1);
       for (uint64_t e = 0; e <= n_sent; ++e) pent[e] = fp::PatchStrut{sent[e].len, sloc[e], scls[e]};

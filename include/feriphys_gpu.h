@@ -199,9 +199,9 @@ int fp_sim_set_torsion(fp_sim* sim, int32_t enabled);
 /* Kernel selection.  FP_KERNEL_AUTO uses the LDS-tiled grid kernel for meshes built by Cloth::new
  * and left unchanged, and for other meshes the patch kernel (one workgroup per connected patch of
  * up to 256 vertices, patch and halo staged in LDS, faces computed in the same pass) when every
- * patch and halo fits it (at most 512 vertices, at most 8 faces per vertex, at most 16 strut
- * classes; see fp_sim_get_mesh_kernel), the CSR kernel (one
- * thread per vertex, after a face kernel) otherwise.  FP_KERNEL_CSR puts every mesh on the CSR
+ * patch and halo fits it (at most 512 vertices, at most 2,048 faces with a corner in the patch,
+ * at most 16 strut classes; see fp_sim_get_mesh_kernel), the CSR kernel (one thread per vertex,
+ * after a face kernel) otherwise.  FP_KERNEL_CSR puts every mesh on the CSR
  * kernel; FP_KERNEL_GRID_SIMPLE is the one-thread-per-vertex grid kernel without LDS (kept as a
  * cross-check).  All kernels return the same results.  EXACT kernels follow the reference's operation
  * order with no FMA contraction, correctly rounded division and square root and the C library's

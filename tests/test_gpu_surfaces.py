@@ -1,7 +1,7 @@
 """GPU parity on irregular meshes (the patch kernel, and the CSR kernel with its face kernel):
 seeded, radially perturbed icospheres with tensile struts on edges and bending struts across
-shared edges, dropped onto a ground plane; a fan whose hub has more faces than the patch kernel
-takes."""
+shared edges, dropped onto a ground plane; a fan whose hub has twice an icosphere vertex's faces;
+the per-mesh choice between the two kernels."""
 import numpy as np
 import pytest
 
@@ -57,10 +57,11 @@ def test_icosphere_one_step_forces_match_oracle():
         assert (bits(g.get_points(k)) == bits(o.mesh(k).points())).all()
 
 
-def test_fan_with_many_faces_per_vertex_matches_oracle():
-    """A hub with 12 faces (more than the patch kernel holds per vertex) keeps the mesh on the CSR
-    kernel; an icosphere in the same simulation is on the patch kernel.  Both bit-identical to the
-    oracle over 120 steps with wind, drag and lift."""
+@pytest.mark.parametrize("kernel", [fa.FP_KERNEL_AUTO, fa.FP_KERNEL_CSR])
+def test_fan_with_many_faces_per_vertex_matches_oracle(kernel):
+    """A hub vertex with 12 faces (twice an icosphere vertex's) and an icosphere in one simulation,
+    on the patch kernel and on the CSR kernel: bit-identical to the oracle over 120 steps with wind,
+    drag and lift."""
     k = 12
     ang = np.arange(k) * (2 * np.pi / k)
     fan_pos = np.concatenate([[[0.0, 0.3, 0.0]], np.stack([0.2 * np.cos(ang), 0.3 + 0.01 * np.sin(3 * ang),
@@ -68,8 +69,11 @@ def test_fan_with_many_faces_per_vertex_matches_oracle():
     fan_idx = np.array([[0, 1 + j, 1 + (j + 1) % k] for j in range(k)], np.uint64).reshape(-1)
     sc = wl.config4(n_meshes=1, seed=7, levels=[3])
     sc.meshes.append(wl.SurfaceSpec(fan_pos, fan_idx, 0.05, 5.0, 0.02, 0.5, 0.002))
-    announce(f"fan of {k} faces + level-3 icosphere")
+    announce(f"fan of {k} faces + level-3 icosphere, kernel {kernel}")
     g = wl.build_surface_gpu(sc)
+    g.set_kernel(kernel, fa.FP_MATH_EXACT)
+    expect = fa.FP_STEP_PATCH if kernel == fa.FP_KERNEL_AUTO else fa.FP_STEP_CSR
+    assert [g.get_mesh_kernel(m) for m in range(2)] == [expect, expect]
     o = orc.build_surface_scene(sc)
     g.step(120)
     o.step(120)
