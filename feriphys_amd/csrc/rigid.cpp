@@ -24,7 +24,8 @@ struct fp_rigid_sim {
   uint64_t n = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DBuf<float> rec[2];  // ping-pong, kRigidStride floats per body
+  uint64_t pitch = 0;  // floats per field of the device records (n rounded up to kRigidAlign)
+  DBuf<float> rec[2];  // ping-pong, field-major (rigid.hpp)
   int cur = 0;
   fph::ObstacleSet obst;
   fp_rigid_config cfg{};
@@ -166,8 +167,10 @@ int fp_rigid_sim_new(fp_rigid* const* bodies, uint64_t n_bodies, fp_obstacle* co
   HIP_TRY(hipEventCreate(&s->ev1));
   fp_rigid_config_default(&s->cfg);
   s->n = n_bodies;
-  std::vector<float> h((size_t)fp::kRigidStride * n_bodies + 1, 0.0f);
-  for (uint64_t i = 0; i < n_bodies; ++i) std::memcpy(&h[(size_t)fp::kRigidStride * i], bs[i]->rec, sizeof(bs[i]->rec));
+  s->pitch = (n_bodies + fp::kRigidAlign - 1) / fp::kRigidAlign * fp::kRigidAlign;
+  std::vector<float> h(fp::kRigidRecord * s->pitch + 1, 0.0f);
+  for (uint64_t i = 0; i < n_bodies; ++i)
+    for (int k = 0; k < fp::kRigidRecord; ++k) h[k * s->pitch + i] = bs[i]->rec[k];
   HIP_TRY(s->rec[0].upload(h.data(), h.size()));
   HIP_TRY(s->rec[1].upload(h.data(), h.size()));
   std::vector<fp::HostObstacle> obs;
@@ -202,7 +205,8 @@ int fp_rigid_sim_get_config(const fp_rigid_sim* s, fp_rigid_config* c) {
 static int run_rigid(fp_rigid_sim* s, uint64_t n) {
   const fp::RigidParams p = params_of(s->cfg);
   for (uint64_t k = 0; k < n; ++k) {
-    HIP_TRY(fp::launch_rigid_step(s->rec[s->cur].p, s->rec[s->cur ^ 1].p, (uint32_t)s->n, p, s->obst.view, s->stream));
+    HIP_TRY(fp::launch_rigid_step(s->rec[s->cur].p, s->rec[s->cur ^ 1].p, (uint32_t)s->n, (uint32_t)s->pitch, p,
+                                  s->obst.view, s->stream));
     s->cur ^= 1;
   }
   return FP_OK;
@@ -229,10 +233,10 @@ int fp_rigid_sim_time_steps(fp_rigid_sim* s, uint64_t n_steps, float* ms) {
 int fp_rigid_sim_get_states(fp_rigid_sim* s, float* records) {
   if (!s || (!records && s->n)) return fail(FP_EINVAL, "NULL argument");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<float> h((size_t)fp::kRigidStride * s->n);
+  std::vector<float> h(fp::kRigidRecord * s->pitch);
   if (s->n) HIP_TRY(hipMemcpy(h.data(), s->rec[s->cur].p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
   for (uint64_t i = 0; i < s->n; ++i)
-    std::memcpy(records + (size_t)fp::kRigidRecord * i, &h[(size_t)fp::kRigidStride * i], fp::kRigidRecord * sizeof(float));
+    for (int k = 0; k < fp::kRigidRecord; ++k) records[(size_t)fp::kRigidRecord * i + k] = h[k * s->pitch + i];
   return FP_OK;
 }
 
@@ -241,10 +245,10 @@ int fp_rigid_sim_set_states(fp_rigid_sim* s, const float* records, uint64_t n) {
   if (n != s->n) return fail(FP_EINVAL, "Incorrect size of state vector! (record count != body count)");
   if (n && !records) return fail(FP_EINVAL, "records is NULL");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<float> h((size_t)fp::kRigidStride * n + 1, 0.0f);
+  std::vector<float> h(fp::kRigidRecord * s->pitch, 0.0f);
   for (uint64_t i = 0; i < n; ++i)
-    std::memcpy(&h[(size_t)fp::kRigidStride * i], records + (size_t)fp::kRigidRecord * i, fp::kRigidRecord * sizeof(float));
-  HIP_TRY(hipMemcpy(s->rec[s->cur].p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    for (int k = 0; k < fp::kRigidRecord; ++k) h[k * s->pitch + i] = records[(size_t)fp::kRigidRecord * i + k];
+  if (n) HIP_TRY(hipMemcpy(s->rec[s->cur].p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
   return FP_OK;
 }
 
@@ -253,10 +257,11 @@ int fp_rigid_sim_apply_impulse(fp_rigid_sim* s, uint64_t body, const float impul
   if (body >= s->n) return fail(FP_EINVAL, "body index out of range");
   HIP_TRY(hipStreamSynchronize(s->stream));
   float r[fp::kRigidRecord];
-  float* d = s->rec[s->cur].p + (size_t)fp::kRigidStride * body;
-  HIP_TRY(hipMemcpy(r, d, sizeof(r), hipMemcpyDeviceToHost));
+  float* d = s->rec[s->cur].p + body;  // field k at d + k * pitch
+  const size_t fp_bytes = s->pitch * sizeof(float);
+  HIP_TRY(hipMemcpy2D(r, sizeof(float), d, fp_bytes, sizeof(float), fp::kRigidRecord, hipMemcpyDeviceToHost));
   apply_impulse(r, impulse, position);
-  HIP_TRY(hipMemcpy(d, r, sizeof(r), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy2D(d, fp_bytes, r, sizeof(float), sizeof(float), fp::kRigidRecord, hipMemcpyHostToDevice));
   return FP_OK;
 }
 

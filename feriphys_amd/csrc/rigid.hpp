@@ -8,9 +8,11 @@ namespace fp {
 
 // A body's record: the reference's 29-float Stateful vector (rigidbody.rs:62-93: position,
 // rotation v then s, linear momentum, angular momentum, mass, initial inverse inertia by columns,
-// accumulated force, accumulated torque), stored in 32-float (128-byte) strides.
+// accumulated force, accumulated torque).  On the device the records are stored field-major:
+// field k of body b at k * pitch + b (pitch = the body count rounded up to kRigidAlign), so that
+// a wavefront's load of one field reads consecutive floats.
 constexpr int kRigidRecord = 29;
-constexpr int kRigidStride = 32;
+constexpr uint64_t kRigidAlign = 64;
 
 // The vertices of the unit cube every RigidBody::new builds (graphics/forms.rs:91-146), in its order.
 constexpr float kCubeCorner[8][3] = {{-0.5f, -0.5f, 0.5f}, {0.5f, -0.5f, 0.5f},   {0.5f, 0.5f, 0.5f},
@@ -27,8 +29,8 @@ struct RigidParams {
   f3 torque;   // added to the accumulated torque each step
 };
 
-// One Simulation::step of n bodies: records in -> out (kRigidStride floats per body).
-hipError_t launch_rigid_step(const float* in, float* out, uint32_t n, const RigidParams& p, const ObsView& obs,
-                             hipStream_t s);
+// One Simulation::step of n bodies: records in -> out (field-major, `pitch` floats per field).
+hipError_t launch_rigid_step(const float* in, float* out, uint32_t n, uint32_t pitch, const RigidParams& p,
+                             const ObsView& obs, hipStream_t s);
 
 }  // namespace fp

@@ -72,17 +72,13 @@ __device__ __forceinline__ void derivative(const float* S, float* D) {
 }
 
 __global__ void __launch_bounds__(kBlock) rigid_step_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                                             uint32_t n, RigidParams P, ObsView O) {
+                                                             uint32_t n, uint32_t pitch, RigidParams P, ObsView O) {
   const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
   if (b >= n) return;
-  float S[kRigidStride], T[kRigidStride], K[kRigidRecord], A[kRigidRecord];
-  // the 128-byte record as eight 16-byte loads (one wavefront load instruction per 16 bytes)
-  const float4* src = reinterpret_cast<const float4*>(in + (size_t)kRigidStride * b);
+  float S[kRigidRecord], T[kRigidRecord], K[kRigidRecord], A[kRigidRecord];
+  // field-major records: each field is one coalesced wavefront load
 #pragma unroll
-  for (int q = 0; q < kRigidStride / 4; ++q) {
-    const float4 v = src[q];
-    S[4 * q] = v.x; S[4 * q + 1] = v.y; S[4 * q + 2] = v.z; S[4 * q + 3] = v.w;
-  }
+  for (int k = 0; k < kRigidRecord; ++k) S[k] = in[(size_t)k * pitch + b];
   // accumulate_forces, accumulate_torques
   S[23] = S[23] + P.gravity.x; S[24] = S[24] + P.gravity.y; S[25] = S[25] + P.gravity.z;
   S[26] = S[26] + P.torque.x; S[27] = S[27] + P.torque.y; S[28] = S[28] + P.torque.z;
@@ -141,20 +137,20 @@ __global__ void __launch_bounds__(kBlock) rigid_step_kernel(const float* __restr
       T[10] = T[10] + dl.x; T[11] = T[11] + dl.y; T[12] = T[12] + dl.z;
     }
   }
-  // clear_forces, clear_torques (and the record's padding)
+  // clear_forces, clear_torques
 #pragma unroll
-  for (int k = 23; k < kRigidStride; ++k) T[k] = 0.0f;
-  float4* dst = reinterpret_cast<float4*>(out + (size_t)kRigidStride * b);
+  for (int k = 23; k < kRigidRecord; ++k) T[k] = 0.0f;
 #pragma unroll
-  for (int q = 0; q < kRigidStride / 4; ++q) dst[q] = make_float4(T[4 * q], T[4 * q + 1], T[4 * q + 2], T[4 * q + 3]);
+  for (int k = 0; k < kRigidRecord; ++k) out[(size_t)k * pitch + b] = T[k];
 }
 
 }  // namespace
 
-hipError_t launch_rigid_step(const float* in, float* out, uint32_t n, const RigidParams& p, const ObsView& obs,
-                             hipStream_t s) {
+hipError_t launch_rigid_step(const float* in, float* out, uint32_t n, uint32_t pitch, const RigidParams& p,
+                             const ObsView& obs, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  rigid_step_kernel<<<dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s>>>(in, out, n, p, obs);
+  if (pitch < n) return hipErrorInvalidValue;
+  rigid_step_kernel<<<dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s>>>(in, out, n, pitch, p, obs);
   return hipGetLastError();
 }
 
