@@ -111,14 +111,14 @@ __global__ void __launch_bounds__(kBlock) csr_kernel(StepArgs A, CsrView T, cons
 //   phase 2  every patch vertex: struts in strut order (8-byte entries, neighbour states from LDS),
 //            then its faces in face order, each its own corner's weight (angle / PI) times the
 //            face's drag + lift; integrate, collide, pin, store.
-// LDS is sized per launch: 6 x nl_cap floats of state and 3 x nf_cap of face forces.
+// LDS: the staged states as (x, y, z, vx) and (vy, vz) at fixed strides (one address per
+// neighbour, two wide reads), and the face forces sized per launch, nf_cap float4.
 template <class M>
-__global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, const PatchRange* __restrict__ patches,
-                                                        uint32_t nl_cap, uint32_t nf_cap) {
-  extern __shared__ float s_dyn[];
-  float* const s_st = s_dyn;                // [6][nl_cap]
-  float* const s_ff = s_dyn + 6 * nl_cap;   // [3][nf_cap]
+__global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, const PatchRange* __restrict__ patches) {
+  __shared__ float4 s_p[kPatchLocal];  // x, y, z, vx
+  __shared__ float2 s_w[kPatchLocal];  // vy, vz
   __shared__ float2 s_cls[kPatchClasses];
+  extern __shared__ float4 s_ff[];     // per patch face: drag + lift
   const PatchRange R = patches[blockIdx.x];
   const uint32_t np = R.end - R.begin, nl = np + R.n_halo;
   const SoA& S = A.in;
@@ -126,16 +126,16 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, co
   if (threadIdx.x < kPatchClasses) s_cls[threadIdx.x] = T.p_cls[R.mesh * kPatchClasses + threadIdx.x];
   for (uint32_t e = threadIdx.x; e < nl; e += kBlock) {
     const uint32_t g = e < np ? R.begin + e : T.halo[R.halo_begin + (e - np)];
-    s_st[e] = S.x[g]; s_st[nl_cap + e] = S.y[g]; s_st[2 * nl_cap + e] = S.z[g];
-    s_st[3 * nl_cap + e] = S.vx[g]; s_st[4 * nl_cap + e] = S.vy[g]; s_st[5 * nl_cap + e] = S.vz[g];
+    s_p[e] = make_float4(S.x[g], S.y[g], S.z[g], S.vx[g]);
+    s_w[e] = make_float2(S.vy[g], S.vz[g]);
   }
   __syncthreads();
-  auto Pl = [&](uint32_t l) { return f3{s_st[l], s_st[nl_cap + l], s_st[2 * nl_cap + l]}; };
-  auto Vl = [&](uint32_t l) { return f3{s_st[3 * nl_cap + l], s_st[4 * nl_cap + l], s_st[5 * nl_cap + l]}; };
+  auto Pl = [&](uint32_t l) { const float4 a = s_p[l]; return f3{a.x, a.y, a.z}; };
+  auto Vl = [&](uint32_t l) { const float4 a = s_p[l]; const float2 b = s_w[l]; return f3{a.w, b.x, b.y}; };
   for (uint32_t q = threadIdx.x; q < R.n_faces; q += kBlock) {
     const PatchFace pf = T.patch_faces[R.face_begin + q];
     const f3 sum = face_force<M>(Pl(pf.v[0]), Pl(pf.v[1]), Pl(pf.v[2]), Vl(pf.v[0]), Vl(pf.v[1]), Vl(pf.v[2]), P);
-    s_ff[q] = sum.x; s_ff[nf_cap + q] = sum.y; s_ff[2 * nf_cap + q] = sum.z;
+    s_ff[q] = make_float4(sum.x, sum.y, sum.z, 0.0f);
   }
   __syncthreads();
   if (threadIdx.x >= np) return;
@@ -188,8 +188,8 @@ __global__ void __launch_bounds__(kBlock) patch_kernel(StepArgs A, CsrView T, co
       for (int k = 0; k < 4; ++k)
         if (j + k < nf) {
           const float w = corner_weight<M>(p, Pl(ce[k].next), Pl(ce[k].prev));
-          const uint32_t q = ce[k].q;
-          F = add(F, f3{w * s_ff[q], w * s_ff[nf_cap + q], w * s_ff[2 * nf_cap + q]});
+          const float4 ff = s_ff[ce[k].q];
+          F = add(F, f3{w * ff.x, w * ff.y, w * ff.z});
         }
     }
   }
@@ -420,12 +420,12 @@ hipError_t launch_selftest(unsigned long long* bad, hipStream_t s) {
 }
 
 hipError_t launch_patches(const StepArgs& a, const CsrView& t, const PatchRange* patches, uint32_t n_patches,
-                          uint32_t nl_cap, uint32_t nf_cap, bool exact, hipStream_t s) {
+                          uint32_t nf_cap, bool exact, hipStream_t s) {
   if (n_patches == 0) return hipSuccess;
-  if (nl_cap == 0 || nl_cap > kPatchLocal || nf_cap > kPatchFaceCap) return hipErrorInvalidValue;
-  const size_t lds = sizeof(float) * (6 * (size_t)nl_cap + 3 * (size_t)nf_cap);
-  if (exact) patch_kernel<ExactMath><<<dim3(n_patches), dim3(kBlock), lds, s>>>(a, t, patches, nl_cap, nf_cap);
-  else patch_kernel<FastMath><<<dim3(n_patches), dim3(kBlock), lds, s>>>(a, t, patches, nl_cap, nf_cap);
+  if (nf_cap > kPatchFaceCap) return hipErrorInvalidValue;
+  const size_t lds = sizeof(float4) * (size_t)nf_cap;
+  if (exact) patch_kernel<ExactMath><<<dim3(n_patches), dim3(kBlock), lds, s>>>(a, t, patches);
+  else patch_kernel<FastMath><<<dim3(n_patches), dim3(kBlock), lds, s>>>(a, t, patches);
   return hipGetLastError();
 }
 

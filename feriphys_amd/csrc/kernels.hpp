@@ -133,10 +133,10 @@ hipError_t launch_nonfinite(const SoA& state, const BlockRange* blocks, uint32_t
 // operations they replace (bad[0..4], added to; see selftest_kernel).
 hipError_t launch_selftest(unsigned long long* bad, hipStream_t s);
 // The step (MODE_STEP) of general meshes one patch per workgroup, the neighbour states read from
-// the patch's LDS copy; the same results as launch_csr.  nl_cap and nf_cap bound every patch's
-// vertices plus halo (<= kPatchLocal) and faces (<= kPatchFaceCap): they size its LDS.
+// the patch's LDS copy; the same results as launch_csr.  nf_cap bounds every patch's faces
+// (<= kPatchFaceCap) and sizes the face part of its LDS.
 hipError_t launch_patches(const StepArgs& a, const CsrView& t, const PatchRange* patches, uint32_t n_patches,
-                          uint32_t nl_cap, uint32_t nf_cap, bool exact, hipStream_t s);
+                          uint32_t nf_cap, bool exact, hipStream_t s);
 // Drag + lift and corner weights of the faces in `face_blocks` (BlockRange over global face ids),
 // for the CSR kernel that runs after it in the same stream.
 hipError_t launch_faces(const StepArgs& a, const CsrView& t, const BlockRange* face_blocks, uint32_t n_blocks,

@@ -190,7 +190,7 @@ struct fp_sim {
   DBuf<fp::PatchCorner> p_corner;
   DBuf<fp::PatchRange> patch_list;
   uint32_t n_patch_list = 0;
-  uint32_t patch_nl_cap = 1, patch_nf_cap = 0;  // largest vertices + halo, faces of a listed patch
+  uint32_t patch_nf_cap = 0;  // the most faces of a listed patch
   // face blocks of the meshes on the CSR kernel (BlockRange over global face ids), all meshes'
   // face blocks (for the split API, indexed per mesh by fblk_begin/fblk_end)
   DBuf<fp::BlockRange> csr_face_blocks, all_face_blocks;
@@ -390,12 +390,8 @@ int build_block_lists(fp_sim* s) {
   c = xcd_order(c);
   pl = xcd_order(pl);
   s->n_patch_list = (uint32_t)pl.size();
-  s->patch_nl_cap = 1;
   s->patch_nf_cap = 0;
-  for (const auto& r : pl) {
-    s->patch_nl_cap = std::max(s->patch_nl_cap, r.end - r.begin + r.n_halo);
-    s->patch_nf_cap = std::max(s->patch_nf_cap, r.n_faces);
-  }
+  for (const auto& r : pl) s->patch_nf_cap = std::max(s->patch_nf_cap, r.n_faces);
   HIP_TRY(s->patch_list.upload(pl.data(), pl.size()));
   cf = xcd_order(cf);
   sb = xcd_order(sb);
@@ -444,8 +440,7 @@ int run_steps(fp_sim* s, uint64_t n) {
     if (s->torsion_on)
       HIP_TRY(fp::launch_hinges(a, s->csr(), s->hinges.p, 0, s->n_hinges, exact, s->stream));
     HIP_TRY(fp::launch_csr(a, s->csr(), s->csr_blocks.p, s->n_csr_blocks, fp::MODE_STEP, exact, s->stream));
-    HIP_TRY(fp::launch_patches(a, s->csr(), s->patch_list.p, s->n_patch_list, s->patch_nl_cap, s->patch_nf_cap,
-                               exact, s->stream));
+    HIP_TRY(fp::launch_patches(a, s->csr(), s->patch_list.p, s->n_patch_list, s->patch_nf_cap, exact, s->stream));
     s->cur ^= 1;
     s->forces_dirty = false;
   }
