@@ -30,17 +30,31 @@ __device__ __forceinline__ StrutK strut_k(float dX, float dY, float kn, float cn
   return StrutK{inv * kn, inv * cn, L};
 }
 
-template <class M, int TR, int TC, int MODE>
-__global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const ClothView* __restrict__ cloths,
-                                                             const TileRange* __restrict__ tiles) {
-  constexpr int TH = TR + 4, TW = TC + 4;  // staged rows/cols: r0-2 .. r0+TR+1, c0-2 .. c0+TC+1
-  constexpr int QR = TR + 1, QC = TC + 1;  // quads (r', c') with r' in [r0-1, r0+TR), c' in [c0-1, c0+TC)
-  __shared__ float s_st[6][TH][TW];
-  __shared__ float s_face[2][QR][QC][3];  // drag + lift
-  __shared__ StrutK s_shear[QR][QC];
-  __shared__ StrutK s_h[TC + 1], s_v[TR + 1], s_bh[TC + 2], s_bv[TR + 2];
+// The LDS of one TR x TC tile: state tile + halo (rows r0-2 .. r0+TR+1, cols c0-2 .. c0+TC+1), the
+// drag + lift of the faces of quads (r', c') with r' in [r0-1, r0+TR), c' in [c0-1, c0+TC), and the
+// strut constant tables.
+template <int TR, int TC>
+struct TileLds {
+  float st[6][TR + 4][TC + 4];
+  float face[2][TR + 1][TC + 1][3];
+  StrutK shear[TR + 1][TC + 1];
+  StrutK h[TC + 1], v[TR + 1], bh[TC + 2], bv[TR + 2];
+};
 
-  const TileRange T = tiles[blockIdx.x];
+// One tile's step by one workgroup (all of its threads call it).
+template <class M, int TR, int TC, int MODE>
+__device__ __forceinline__ void tile_step(TileLds<TR, TC>& L, const StepArgs& A, const ClothView* __restrict__ cloths,
+                                          const TileRange T) {
+  constexpr int TH = TR + 4, TW = TC + 4;
+  constexpr int QR = TR + 1, QC = TC + 1;
+  auto& s_st = L.st;
+  auto& s_face = L.face;
+  auto& s_shear = L.shear;
+  auto& s_h = L.h;
+  auto& s_v = L.v;
+  auto& s_bh = L.bh;
+  auto& s_bv = L.bv;
+
   const StepParams P = A.params[T.mesh];
   const ClothView G = cloths[T.mesh];
   const int R = (int)G.rows, C = (int)G.cols;
@@ -169,6 +183,30 @@ __global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const Cl
   }
 }
 
+template <class M, int TR, int TC, int MODE>
+__global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const ClothView* __restrict__ cloths,
+                                                             const TileRange* __restrict__ tiles) {
+  __shared__ TileLds<TR, TC> L;
+  tile_step<M, TR, TC, MODE>(L, A, cloths, tiles[blockIdx.x]);
+}
+
+// Tiles of every shape in one launch: each workgroup takes its tile's shape, in one LDS block the
+// size of the largest shape's.
+union AnyTileLds {
+  TileLds<15, 15> t15x15;
+  TileLds<7, 31> t7x31;
+  TileLds<16, 16> t16x16;
+};
+template <class M>
+__global__ void __launch_bounds__(kBlock) grid_tiled_multi_kernel(StepArgs A, const ClothView* __restrict__ cloths,
+                                                                   const TileRange* __restrict__ tiles) {
+  __shared__ AnyTileLds L;
+  const TileRange T = tiles[blockIdx.x];
+  if (T.shape == TILE_15x15) tile_step<M, 15, 15, MODE_STEP>(L.t15x15, A, cloths, T);
+  else if (T.shape == TILE_7x31) tile_step<M, 7, 31, MODE_STEP>(L.t7x31, A, cloths, T);
+  else tile_step<M, 16, 16, MODE_STEP>(L.t16x16, A, cloths, T);
+}
+
 template <class M, int TR, int TC>
 hipError_t launch_shape(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n, int mode,
                         hipStream_t s) {
@@ -196,6 +234,14 @@ hipError_t launch_grid_tiled(const StepArgs& a, const ClothView* cloths, const T
   if (n_tiles == 0) return hipSuccess;
   return exact ? launch_math<ExactMath>(a, cloths, tiles, n_tiles, shape, mode, s)
                : launch_math<FastMath>(a, cloths, tiles, n_tiles, shape, mode, s);
+}
+
+hipError_t launch_grid_tiled_multi(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n_tiles,
+                                   bool exact, hipStream_t s) {
+  if (n_tiles == 0) return hipSuccess;
+  if (exact) grid_tiled_multi_kernel<ExactMath><<<dim3(n_tiles), dim3(kBlock), 0, s>>>(a, cloths, tiles);
+  else grid_tiled_multi_kernel<FastMath><<<dim3(n_tiles), dim3(kBlock), 0, s>>>(a, cloths, tiles);
+  return hipGetLastError();
 }
 
 }  // namespace fp
