@@ -125,8 +125,11 @@ def test_grid_and_csr_kernels_identical(rows, cols):
     last tiles."""
     sc = wl.config0(steps=100, rows=rows, cols=cols, pins="edge")
     outs = []
+    expect = {fa.FP_KERNEL_GRID: fa.FP_STEP_TILED, fa.FP_KERNEL_GRID_SIMPLE: fa.FP_STEP_GRID_SIMPLE,
+              fa.FP_KERNEL_CSR: fa.FP_STEP_CSR}
     for k in (fa.FP_KERNEL_GRID, fa.FP_KERNEL_GRID_SIMPLE, fa.FP_KERNEL_CSR):
         g = gpu_scene(sc, k, fa.FP_MATH_EXACT)
+        assert g.get_mesh_kernel(0) == expect[k]
         g.step(sc.steps)
         outs.append(g.get_points(0))
     assert np.isfinite(outs[0]).all()
