@@ -6,7 +6,7 @@ include/feriphys_gpu.h); this package is a thin ctypes mirror of the reference's
 from ._lib import (FP_EULER, FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_KERNEL_GRID_SIMPLE, FP_MATH_EXACT,
                    FP_MATH_FAST,
                    FP_PLANE_XY, FP_PLANE_XZ, FP_RK4, FP_STEP_CSR, FP_STEP_GRID_SIMPLE, FP_STEP_PATCH, FP_STEP_TILED,
-                   FpError, lib)
+                   FP_TILE_7x31, FP_TILE_15x15, FP_TILE_16x16, FpError, lib)
 from .springy import (Cloth, CollidableMesh, Config, Integration, Simulation, SpringyMesh, config_default,
                       selftest_exact_math)
 from .rigid import RigidBody, RigidConfig, RigidSimulation, rigid_config_default
@@ -20,4 +20,5 @@ __all__ = [
     "lib", "FP_EULER", "FP_RK4", "FP_PLANE_XY", "FP_PLANE_XZ", "FP_KERNEL_AUTO", "FP_KERNEL_CSR", "FP_KERNEL_GRID",
     "FP_KERNEL_GRID_SIMPLE",
     "FP_MATH_EXACT", "FP_MATH_FAST", "FP_STEP_TILED", "FP_STEP_GRID_SIMPLE", "FP_STEP_PATCH", "FP_STEP_CSR",
+    "FP_TILE_15x15", "FP_TILE_7x31", "FP_TILE_16x16",
 ]

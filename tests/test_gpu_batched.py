@@ -1,6 +1,6 @@
 """GPU parity of the batched path: many cloths of uneven sizes, each with its own Config, packed
-into one simulation and stepped by one tiled launch per tile shape; HIP-graph replays of captured
-step chunks; the tile-shape autotuner."""
+into one simulation and stepped by the tiled kernel (one launch for tiles of every shape); HIP-graph
+replays of captured step chunks; the tile-shape autotuner."""
 import numpy as np
 import pytest
 
@@ -26,6 +26,26 @@ def test_batched_cloths_match_per_cloth_oracle():
         o.step(sc.steps)
         a, b = g.get_points(k), o.mesh(0).points()
         assert (bits(a) == bits(b)).all(), f"cloth {k} ({sc.cloths[k].rows}x{sc.cloths[k].cols}) differs"
+
+
+def test_all_tile_shapes_in_one_launch_match_oracle():
+    """Cloths given the three tile shapes in turn step in one launch of the multi-shape kernel (each
+    workgroup its own tile's shape) and stay bit-identical to the per-cloth oracle, also after the
+    shapes are switched mid-run."""
+    sc = small_batch(n=18, seed=9, steps=50)
+    g = wl.build_gpu(sc)
+    shapes = [fa.FP_TILE_15x15, fa.FP_TILE_7x31, fa.FP_TILE_16x16]
+    for k in range(len(sc.cloths)):
+        g.set_tile_shape(k, shapes[k % 3])
+    assert all(g.get_mesh_kernel(k) == fa.FP_STEP_TILED for k in range(len(sc.cloths)))
+    sims = orc.build_scene_per_mesh(sc)
+    g.step(25)
+    for k in range(len(sc.cloths)):
+        g.set_tile_shape(k, shapes[(k + 1) % 3])
+    g.step(sc.steps - 25)
+    for k, o in enumerate(sims):
+        o.step(sc.steps)
+        assert (bits(g.get_points(k)) == bits(o.mesh(0).points())).all(), k
 
 
 def test_batched_mixed_kernels_match_oracle():
