@@ -129,6 +129,19 @@ def main():
 
     sur = surrogate_leg(args, np, fa, wl, kern, pts, initial, N, dt, sim_s_per_s)
 
+    # PCIe-inclusive rate (never the headline): a caller that reads the 10-float records back after
+    # every step, as the reference's demo renders every frame; host clock over 20 frames
+    sim.set_points(0, initial)
+    sim.synchronize()
+    frames = 20
+    tr = time.perf_counter()
+    for _ in range(frames):
+        sim.step(1)
+        sim.get_points(0)
+    frame_s = (time.perf_counter() - tr) / frames
+    with_readback = {"ms_per_frame": frame_s * 1e3, "sim_s_per_s": dt / frame_s,
+                     "readback_bytes_per_frame": 40 * N, "frames": frames}
+
     cpu = None
     if args.cpu_steps > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -178,6 +191,7 @@ def main():
                      "traffic": MEASURED_TRAFFIC.get((args.n, args.math, args.kernel, args.tile)),
                      "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": BYTES_PER_VERTEX_STEP * N},
         "valu_issue": MEASURED_VALU.get((args.n, args.math, args.kernel, args.tile)),
+        "with_readback": with_readback,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

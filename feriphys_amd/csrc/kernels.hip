@@ -355,6 +355,24 @@ __global__ void __launch_bounds__(kBlock) update_points_kernel(StepArgs A, const
   A.fx[i] = Fn.x; A.fy[i] = Fn.y; A.fz[i] = Fn.z;
 }
 
+// The Stateful records [m, x, y, z, vx, vy, vz, Fx, Fy, Fz] (springy_mesh.rs:242-255) of device
+// vertices [base, base + n) in the reference's vertex order: record i from device slot
+// base + pos[i] (base + i without a permutation).
+__global__ void __launch_bounds__(kBlock) pack_records_kernel(SoA S, const float* __restrict__ mass,
+                                                               const float* __restrict__ fx, const float* __restrict__ fy,
+                                                               const float* __restrict__ fz,
+                                                               const uint32_t* __restrict__ pos, uint32_t base,
+                                                               uint32_t n, float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = base + (pos != nullptr ? pos[i] : i);
+  float* o = out + 10ull * i;
+  o[0] = mass[d];
+  o[1] = S.x[d]; o[2] = S.y[d]; o[3] = S.z[d];
+  o[4] = S.vx[d]; o[5] = S.vy[d]; o[6] = S.vz[d];
+  o[7] = fx[d]; o[8] = fy[d]; o[9] = fz[d];
+}
+
 // Per mesh, the number of vertices whose position or velocity is not finite.
 __global__ void __launch_bounds__(kBlock) nonfinite_kernel(SoA S, const BlockRange* __restrict__ blocks,
                                                             unsigned long long* __restrict__ counts) {
@@ -455,6 +473,14 @@ hipError_t launch_update_points(const StepArgs& a, const float* nw_rec, uint32_t
                                 const BlockRange* blocks, uint32_t n_blocks, hipStream_t s) {
   if (n_blocks == 0) return hipSuccess;
   update_points_kernel<<<dim3(n_blocks), dim3(kBlock), 0, s>>>(a, nw_rec, rec_base, mass_out, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_records(const SoA& st, const float* mass, const float* fx, const float* fy, const float* fz,
+                               const uint32_t* pos, uint32_t base, uint32_t n, float* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  pack_records_kernel<<<dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s>>>(st, mass, fx, fy, fz, pos, base, n,
+                                                                               out);
   return hipGetLastError();
 }
 

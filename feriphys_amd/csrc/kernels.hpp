@@ -163,6 +163,10 @@ hipError_t launch_update_points(const StepArgs& a, const float* nw_rec, uint32_t
                                 const BlockRange* blocks,
                                 uint32_t n_blocks, hipStream_t s);
 hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s);
+// The 10-float Stateful records of device vertices [base, base + n) into out (10 n floats), record
+// i from device slot base + pos[i] (pos == nullptr: base + i).
+hipError_t launch_pack_records(const SoA& st, const float* mass, const float* fx, const float* fy, const float* fz,
+                               const uint32_t* pos, uint32_t base, uint32_t n, float* out, hipStream_t s);
 
 constexpr uint32_t kBlock = 256;
 

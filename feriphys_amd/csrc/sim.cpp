@@ -175,6 +175,9 @@ struct fp_sim {
   DBuf<float> force[3];
   bool forces_dirty = false;
   DBuf<float> mass;
+  // per general mesh, for every reference vertex its device-local slot (MeshSlot::pos), at vbase
+  DBuf<uint32_t> d_pos;
+  DBuf<float> rec_stage;  // 10-float records packed on the device for one readback
   DBuf<uint32_t> pins;
   DBuf<uint32_t> s_first, s_cnt, f_first, f_cnt, f_v, f_slot;
   DBuf<fp::StrutEnt> s_ent;
@@ -743,6 +746,10 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     for (auto& ms_ : s->meshes)
       for (uint64_t i = 0; i < ms_.host.n; ++i) m[ms_.dev(i)] = ms_.host.rec[10 * i];
     HIP_TRY(s->mass.upload(m.data(), N));
+    std::vector<uint32_t> pos(N, 0);
+    for (auto& ms_ : s->meshes)
+      for (uint64_t i = 0; i < ms_.pos.size(); ++i) pos[ms_.vbase + i] = ms_.pos[i];
+    HIP_TRY(s->d_pos.upload(pos.data(), N));
     for (int k = 0; k < 3; ++k) {
       for (auto& ms_ : s->meshes)
         for (uint64_t i = 0; i < ms_.host.n; ++i) comp[ms_.dev(i)] = ms_.host.rec[10 * i + 7 + k];
@@ -1196,18 +1203,16 @@ int fp_sim_get_points(fp_sim* s, uint64_t mesh, float* rec) {
   if (!rec) return fail(FP_EINVAL, "records is NULL");
   const MeshSlot& m = s->meshes[mesh];
   const uint64_t n = m.host.n, b = m.vbase;
+  if (n == 0) return FP_OK;
+  // the records packed in the reference's order on the device, then one copy
+  HIP_TRY(s->rec_stage.reserve(10 * n));
+  const auto& st = s->state[s->cur];
+  const fp::SoA cur{st[0].p, st[1].p, st[2].p, st[3].p, st[4].p, st[5].p};
+  HIP_TRY(fp::launch_pack_records(cur, s->mass.p, s->force[0].p, s->force[1].p, s->force[2].p,
+                                  m.pos.empty() ? nullptr : s->d_pos.p + b, (uint32_t)b, (uint32_t)n,
+                                  s->rec_stage.p, s->stream));
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<float> comp(n);
-  auto get = [&](const float* d, int slot) -> int {
-    HIP_TRY(hipMemcpy(comp.data(), d + b, n * sizeof(float), hipMemcpyDeviceToHost));
-    for (uint64_t i = 0; i < n; ++i) rec[10 * i + slot] = comp[m.dev(i) - b];
-    return FP_OK;
-  };
-  if (int e = get(s->mass.p, 0)) return e;
-  for (int k = 0; k < 6; ++k)
-    if (int e = get(s->state[s->cur][k].p, 1 + k)) return e;
-  for (int k = 0; k < 3; ++k)
-    if (int e = get(s->force[k].p, 7 + k)) return e;
+  HIP_TRY(hipMemcpy(rec, s->rec_stage.p, 10 * n * sizeof(float), hipMemcpyDeviceToHost));
   return FP_OK;
 }
 
