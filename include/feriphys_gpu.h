@@ -154,7 +154,7 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
 int fp_sim_set_config(fp_sim* sim, const fp_config* config);
 int fp_sim_get_config(const fp_sim* sim, fp_config* config);
 int fp_sim_set_mesh_config(fp_sim* sim, uint64_t mesh, const fp_config* config);
-/* One Config per mesh for all meshes at once (configs[n_meshes]); one upload. */
+/* One Config per mesh for all meshes at once (an array of n_meshes Configs); one upload. */
 int fp_sim_set_mesh_configs(fp_sim* sim, const fp_config* configs, uint64_t n_meshes);
 /* Simulation::step (springy/simulation.rs:27-45), n_steps times; blocks until done.
  * timestep_out (may be NULL) receives Duration::from_secs_f32(dt).as_secs_f32(). */

@@ -187,7 +187,7 @@ def test_non_manifold_edge_panics():
 
 
 def test_corner_pinned_cloth_diverges_in_the_reference_method():
-    """Documents a property of the reference's integrator, not of this build: the 32 x 32 cloth
+    """Documents a property of the reference's integrator, not of the GPU library: the 32 x 32 cloth
     hanging from its two top corners (dt = 1/600, g = 9.81, spacing 0.05) blows up before step 600
     in f32 AND in f64 (so it is the method, not rounding), while the same cloth pinned along its
     whole top row stays bounded."""
