@@ -40,6 +40,22 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.SIGNATURES) == set(syms)
 
 
+def test_rust_binding_declares_every_entry_point():
+    """The appendix of INTEGRATION.md (the extern "C" block a Rust caller adds) declares each
+    function of the header once, with the header's parameter count."""
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = doc[doc.index("## Appendix: every entry point"):]
+    rust = re.findall(r"pub fn (fp_[a-z0-9_]+)\(([^)]*)\)", block)
+    names = [n for n, _ in rust]
+    assert sorted(names) == declared_symbols() and len(names) == len(set(names))
+    txt = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    for name, args in rust:
+        c_args = re.search(r"\b" + name + r"\s*\(([^)]*)\)", txt).group(1).strip()
+        n_c = 0 if c_args in ("", "void") else c_args.count(",") + 1
+        n_r = 0 if not args.strip() else args.count(",") + 1
+        assert n_c == n_r, name
+
+
 def test_config_default():
     # springy/config.rs:20-33
     c = fa.config_default()
