@@ -373,6 +373,22 @@ __global__ void __launch_bounds__(kBlock) pack_records_kernel(SoA S, const float
   o[7] = fx[d]; o[8] = fy[d]; o[9] = fz[d];
 }
 
+// The inverse of pack_records_kernel: records in the reference's vertex order into the state,
+// masses and force records of device vertices [base, base + n).
+__global__ void __launch_bounds__(kBlock) unpack_records_kernel(const float* __restrict__ in, const uint32_t* __restrict__ pos,
+                                                                 uint32_t base, uint32_t n, SoA S, float* __restrict__ mass,
+                                                                 float* __restrict__ fx, float* __restrict__ fy,
+                                                                 float* __restrict__ fz) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = base + (pos != nullptr ? pos[i] : i);
+  const float* q = in + 10ull * i;
+  mass[d] = q[0];
+  S.x[d] = q[1]; S.y[d] = q[2]; S.z[d] = q[3];
+  S.vx[d] = q[4]; S.vy[d] = q[5]; S.vz[d] = q[6];
+  fx[d] = q[7]; fy[d] = q[8]; fz[d] = q[9];
+}
+
 // Per mesh, the number of vertices whose position or velocity is not finite.
 __global__ void __launch_bounds__(kBlock) nonfinite_kernel(SoA S, const BlockRange* __restrict__ blocks,
                                                             unsigned long long* __restrict__ counts) {
@@ -481,6 +497,14 @@ hipError_t launch_pack_records(const SoA& st, const float* mass, const float* fx
   if (n == 0) return hipSuccess;
   pack_records_kernel<<<dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s>>>(st, mass, fx, fy, fz, pos, base, n,
                                                                                out);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_records(const float* in, const uint32_t* pos, uint32_t base, uint32_t n, const SoA& st,
+                                 float* mass, float* fx, float* fy, float* fz, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  unpack_records_kernel<<<dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s>>>(in, pos, base, n, st, mass, fx, fy,
+                                                                                 fz);
   return hipGetLastError();
 }
 

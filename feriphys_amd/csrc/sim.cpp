@@ -1222,20 +1222,19 @@ int fp_sim_set_points(fp_sim* s, uint64_t mesh, const float* rec, uint64_t n) {
   if (n != m.host.n) return fail(FP_EINVAL, "State Vector incorrect size! (record count != point count)");
   if (n && !rec) return fail(FP_EINVAL, "records is NULL");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<float> comp(n);
-  bool nonzero_f = false;
-  auto put = [&](float* d, int slot) -> int {
-    for (uint64_t i = 0; i < n; ++i) comp[m.dev(i) - m.vbase] = rec[10 * i + slot];
-    HIP_TRY(hipMemcpy(d + m.vbase, comp.data(), n * sizeof(float), hipMemcpyHostToDevice));
-    return FP_OK;
-  };
-  if (int e = put(s->mass.p, 0)) return e;
-  for (int k = 0; k < 6; ++k)
-    if (int e = put(s->state[s->cur][k].p, 1 + k)) return e;
-  for (int k = 0; k < 3; ++k) {
-    if (int e = put(s->force[k].p, 7 + k)) return e;
-    for (uint64_t i = 0; i < n && !nonzero_f; ++i) nonzero_f = rec[10 * i + 7 + k] != 0.0f || std::signbit(rec[10 * i + 7 + k]);
+  if (n) {  // one copy, then the records scattered into the device order on the device
+    HIP_TRY(s->rec_stage.reserve(10 * n));
+    HIP_TRY(hipMemcpy(s->rec_stage.p, rec, 10 * n * sizeof(float), hipMemcpyHostToDevice));
+    const auto& st = s->state[s->cur];
+    const fp::SoA cur{st[0].p, st[1].p, st[2].p, st[3].p, st[4].p, st[5].p};
+    HIP_TRY(fp::launch_unpack_records(s->rec_stage.p, m.pos.empty() ? nullptr : s->d_pos.p + m.vbase,
+                                      (uint32_t)m.vbase, (uint32_t)n, cur, s->mass.p, s->force[0].p, s->force[1].p,
+                                      s->force[2].p, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
   }
+  bool nonzero_f = false;
+  for (uint64_t i = 0; i < n && !nonzero_f; ++i)
+    for (int k = 7; k < 10; ++k) nonzero_f = nonzero_f || rec[10 * i + k] != 0.0f || std::signbit(rec[10 * i + k]);
   s->forces_dirty = s->forces_dirty || nonzero_f;
   refresh_mass(m, rec, n);
   if (int e = upload_params(s)) return e;
