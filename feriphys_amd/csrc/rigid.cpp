@@ -26,6 +26,7 @@ struct fp_rigid_sim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   uint64_t pitch = 0;  // floats per field of the device records (n rounded up to kRigidAlign)
   DBuf<float> rec[2];  // ping-pong, field-major (rigid.hpp)
+  DBuf<float> stage;   // the caller's layout (29 floats per body) for one copy each way
   int cur = 0;
   fph::ObstacleSet obst;
   fp_rigid_config cfg{};
@@ -233,10 +234,11 @@ int fp_rigid_sim_time_steps(fp_rigid_sim* s, uint64_t n_steps, float* ms) {
 int fp_rigid_sim_get_states(fp_rigid_sim* s, float* records) {
   if (!s || (!records && s->n)) return fail(FP_EINVAL, "NULL argument");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<float> h(fp::kRigidRecord * s->pitch);
-  if (s->n) HIP_TRY(hipMemcpy(h.data(), s->rec[s->cur].p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
-  for (uint64_t i = 0; i < s->n; ++i)
-    for (int k = 0; k < fp::kRigidRecord; ++k) records[(size_t)fp::kRigidRecord * i + k] = h[k * s->pitch + i];
+  if (s->n == 0) return FP_OK;
+  HIP_TRY(s->stage.reserve(fp::kRigidRecord * s->n));
+  HIP_TRY(fp::launch_rigid_records(s->rec[s->cur].p, s->stage.p, (uint32_t)s->n, (uint32_t)s->pitch, true, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(records, s->stage.p, fp::kRigidRecord * s->n * sizeof(float), hipMemcpyDeviceToHost));
   return FP_OK;
 }
 
@@ -245,10 +247,11 @@ int fp_rigid_sim_set_states(fp_rigid_sim* s, const float* records, uint64_t n) {
   if (n != s->n) return fail(FP_EINVAL, "Incorrect size of state vector! (record count != body count)");
   if (n && !records) return fail(FP_EINVAL, "records is NULL");
   HIP_TRY(hipStreamSynchronize(s->stream));
-  std::vector<float> h(fp::kRigidRecord * s->pitch, 0.0f);
-  for (uint64_t i = 0; i < n; ++i)
-    for (int k = 0; k < fp::kRigidRecord; ++k) h[k * s->pitch + i] = records[(size_t)fp::kRigidRecord * i + k];
-  if (n) HIP_TRY(hipMemcpy(s->rec[s->cur].p, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+  if (n == 0) return FP_OK;
+  HIP_TRY(s->stage.reserve(fp::kRigidRecord * n));
+  HIP_TRY(hipMemcpy(s->stage.p, records, fp::kRigidRecord * n * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(fp::launch_rigid_records(s->stage.p, s->rec[s->cur].p, (uint32_t)n, (uint32_t)s->pitch, false, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
   return FP_OK;
 }
 

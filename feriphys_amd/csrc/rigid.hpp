@@ -29,6 +29,10 @@ struct RigidParams {
   f3 torque;   // added to the accumulated torque each step
 };
 
+// pack: field-major records `in` -> 29 floats per body, body after body, in `out`; otherwise the
+// inverse.
+hipError_t launch_rigid_records(const float* in, float* out, uint32_t n, uint32_t pitch, bool pack, hipStream_t s);
+
 // One Simulation::step of n bodies: records in -> out (field-major, `pitch` floats per field).
 hipError_t launch_rigid_step(const float* in, float* out, uint32_t n, uint32_t pitch, const RigidParams& p,
                              const ObsView& obs, hipStream_t s);

@@ -144,7 +144,32 @@ __global__ void __launch_bounds__(kBlock) rigid_step_kernel(const float* __restr
   for (int k = 0; k < kRigidRecord; ++k) out[(size_t)k * pitch + b] = T[k];
 }
 
+// Field-major device records <-> the caller's 29-float records, body after body.
+__global__ void __launch_bounds__(kBlock) rigid_pack_kernel(const float* __restrict__ in, uint32_t n, uint32_t pitch,
+                                                             float* __restrict__ out) {
+  const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= n) return;
+#pragma unroll
+  for (int k = 0; k < kRigidRecord; ++k) out[(size_t)kRigidRecord * b + k] = in[(size_t)k * pitch + b];
+}
+__global__ void __launch_bounds__(kBlock) rigid_unpack_kernel(const float* __restrict__ in, uint32_t n, uint32_t pitch,
+                                                               float* __restrict__ out) {
+  const uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+  if (b >= n) return;
+#pragma unroll
+  for (int k = 0; k < kRigidRecord; ++k) out[(size_t)k * pitch + b] = in[(size_t)kRigidRecord * b + k];
+}
+
 }  // namespace
+
+hipError_t launch_rigid_records(const float* in, float* out, uint32_t n, uint32_t pitch, bool pack, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (pitch < n) return hipErrorInvalidValue;
+  const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+  if (pack) rigid_pack_kernel<<<g, b, 0, s>>>(in, n, pitch, out);
+  else rigid_unpack_kernel<<<g, b, 0, s>>>(in, n, pitch, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_rigid_step(const float* in, float* out, uint32_t n, uint32_t pitch, const RigidParams& p,
                              const ObsView& obs, hipStream_t s) {
