@@ -190,23 +190,6 @@ __global__ void __launch_bounds__(kBlock) grid_tiled_kernel(StepArgs A, const Cl
   tile_step<M, TR, TC, MODE>(L, A, cloths, tiles[blockIdx.x]);
 }
 
-// Tiles of every shape in one launch: each workgroup takes its tile's shape, in one LDS block the
-// size of the largest shape's.
-union AnyTileLds {
-  TileLds<15, 15> t15x15;
-  TileLds<7, 31> t7x31;
-  TileLds<16, 16> t16x16;
-};
-template <class M>
-__global__ void __launch_bounds__(kBlock) grid_tiled_multi_kernel(StepArgs A, const ClothView* __restrict__ cloths,
-                                                                   const TileRange* __restrict__ tiles) {
-  __shared__ AnyTileLds L;
-  const TileRange T = tiles[blockIdx.x];
-  if (T.shape == TILE_15x15) tile_step<M, 15, 15, MODE_STEP>(L.t15x15, A, cloths, T);
-  else if (T.shape == TILE_7x31) tile_step<M, 7, 31, MODE_STEP>(L.t7x31, A, cloths, T);
-  else tile_step<M, 16, 16, MODE_STEP>(L.t16x16, A, cloths, T);
-}
-
 template <class M, int TR, int TC>
 hipError_t launch_shape(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n, int mode,
                         hipStream_t s) {
@@ -234,14 +217,6 @@ hipError_t launch_grid_tiled(const StepArgs& a, const ClothView* cloths, const T
   if (n_tiles == 0) return hipSuccess;
   return exact ? launch_math<ExactMath>(a, cloths, tiles, n_tiles, shape, mode, s)
                : launch_math<FastMath>(a, cloths, tiles, n_tiles, shape, mode, s);
-}
-
-hipError_t launch_grid_tiled_multi(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n_tiles,
-                                   bool exact, hipStream_t s) {
-  if (n_tiles == 0) return hipSuccess;
-  if (exact) grid_tiled_multi_kernel<ExactMath><<<dim3(n_tiles), dim3(kBlock), 0, s>>>(a, cloths, tiles);
-  else grid_tiled_multi_kernel<FastMath><<<dim3(n_tiles), dim3(kBlock), 0, s>>>(a, cloths, tiles);
-  return hipGetLastError();
 }
 
 }  // namespace fp

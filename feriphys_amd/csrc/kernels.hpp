@@ -153,9 +153,6 @@ hipError_t launch_grid(const StepArgs& a, const ClothView* cloths, const BlockRa
 // LDS-tiled grid kernel over `n_tiles` tiles that all have tile shape `shape`.
 hipError_t launch_grid_tiled(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n_tiles,
                              int shape, int mode, bool exact, hipStream_t s);
-// The step (MODE_STEP) of tiles of any shapes in one launch, each workgroup its tile's shape.
-hipError_t launch_grid_tiled_multi(const StepArgs& a, const ClothView* cloths, const TileRange* tiles,
-                                   uint32_t n_tiles, bool exact, hipStream_t s);
 // SpringyMesh::update_points on device: old state = a.in (+ force records, masses), new = `nw_rec`
 // (10-float Stateful records, indexed by global vertex id); the result is written in place into
 // a.in, the force records and the masses.

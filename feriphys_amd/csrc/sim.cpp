@@ -209,10 +209,6 @@ struct fp_sim {
   // tiles of every tiled mesh in XCD-aware launch order
   DBuf<fp::TileRange> tiles[fp::N_TILE_SHAPES], mesh_tiles;
   uint32_t n_tiles[fp::N_TILE_SHAPES] = {0, 0, 0};
-  // when the cloths use more than one tile shape: all of their tiles for one launch of the
-  // multi-shape kernel, shape by shape, dealt to the XCDs in bands
-  DBuf<fp::TileRange> tiles_multi;
-  uint32_t n_tiles_multi = 0;
   DBuf<fp::ClothView> cloths;
   DBuf<fp::StepParams> params;
   int32_t kernel = FP_KERNEL_AUTO, math = FP_MATH_EXACT;
@@ -413,18 +409,11 @@ int build_block_lists(fp_sim* s) {
   s->n_csr_blocks = (uint32_t)c.size();
   HIP_TRY(s->grid_blocks.upload(g.data(), g.size()));
   HIP_TRY(s->csr_blocks.upload(c.data(), c.size()));
-  int n_shapes = 0;
-  std::vector<fp::TileRange> multi;
   for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh) {
     std::vector<fp::TileRange> o = xcd_order(by_shape[sh]);
     s->n_tiles[sh] = (uint32_t)o.size();
     HIP_TRY(s->tiles[sh].upload(o.data(), o.size()));
-    n_shapes += !by_shape[sh].empty();
-    multi.insert(multi.end(), by_shape[sh].begin(), by_shape[sh].end());
   }
-  multi = n_shapes > 1 ? xcd_order(multi) : std::vector<fp::TileRange>{};
-  s->n_tiles_multi = (uint32_t)multi.size();
-  HIP_TRY(s->tiles_multi.upload(multi.data(), multi.size()));
   HIP_TRY(s->mesh_tiles.upload(all_tiles.data(), all_tiles.size()));
   ++s->epoch;
   return FP_OK;
@@ -446,12 +435,9 @@ int run_steps(fp_sim* s, uint64_t n) {
   for (uint64_t k = 0; k < n; ++k) {
     fp::StepArgs a = s->args(s->forces_dirty);
     bool exact = s->math == FP_MATH_EXACT;
-    if (s->n_tiles_multi > 0)
-      HIP_TRY(fp::launch_grid_tiled_multi(a, s->cloths.p, s->tiles_multi.p, s->n_tiles_multi, exact, s->stream));
-    else
-      for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh)
-        HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->tiles[sh].p, s->n_tiles[sh], sh, fp::MODE_STEP, exact,
-                                      s->stream));
+    for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh)
+      HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->tiles[sh].p, s->n_tiles[sh], sh, fp::MODE_STEP, exact,
+                                    s->stream));
     HIP_TRY(fp::launch_grid(a, s->cloths.p, s->grid_blocks.p, s->n_grid_blocks, fp::MODE_STEP, exact, s->stream));
     HIP_TRY(fp::launch_faces(a, s->csr(), s->csr_face_blocks.p, s->n_csr_face_blocks, exact, s->stream));
     if (s->torsion_on)

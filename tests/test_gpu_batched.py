@@ -1,6 +1,6 @@
 """GPU parity of the batched path: many cloths of uneven sizes, each with its own Config, packed
-into one simulation and stepped by the tiled kernel (one launch for tiles of every shape); HIP-graph
-replays of captured step chunks; the tile-shape autotuner."""
+into one simulation and stepped by the tiled kernel (one launch per tile shape); HIP-graph replays
+of captured step chunks; the tile-shape autotuner."""
 import numpy as np
 import pytest
 
@@ -28,10 +28,9 @@ def test_batched_cloths_match_per_cloth_oracle():
         assert (bits(a) == bits(b)).all(), f"cloth {k} ({sc.cloths[k].rows}x{sc.cloths[k].cols}) differs"
 
 
-def test_all_tile_shapes_in_one_launch_match_oracle():
-    """Cloths given the three tile shapes in turn step in one launch of the multi-shape kernel (each
-    workgroup its own tile's shape) and stay bit-identical to the per-cloth oracle, also after the
-    shapes are switched mid-run."""
+def test_all_tile_shapes_in_one_step_match_oracle():
+    """Cloths given the three tile shapes in turn (three launches per step) stay bit-identical to
+    the per-cloth oracle, also after the shapes are switched mid-run."""
     sc = small_batch(n=18, seed=9, steps=50)
     g = wl.build_gpu(sc)
     shapes = [fa.FP_TILE_15x15, fa.FP_TILE_7x31, fa.FP_TILE_16x16]
