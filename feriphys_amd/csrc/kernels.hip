@@ -373,6 +373,15 @@ __global__ void __launch_bounds__(kBlock) pack_records_kernel(SoA S, const float
   o[7] = fx[d]; o[8] = fy[d]; o[9] = fz[d];
 }
 
+// The positions of device vertices [base, base + n) in the reference's vertex order, 3 floats each.
+__global__ void __launch_bounds__(kBlock) pack_positions_kernel(SoA S, const uint32_t* __restrict__ pos, uint32_t base,
+                                                                 uint32_t n, float* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = base + (pos != nullptr ? pos[i] : i);
+  out[3ull * i] = S.x[d]; out[3ull * i + 1] = S.y[d]; out[3ull * i + 2] = S.z[d];
+}
+
 // The inverse of pack_records_kernel: records in the reference's vertex order into the state,
 // masses and force records of device vertices [base, base + n).
 __global__ void __launch_bounds__(kBlock) unpack_records_kernel(const float* __restrict__ in, const uint32_t* __restrict__ pos,
@@ -497,6 +506,13 @@ hipError_t launch_pack_records(const SoA& st, const float* mass, const float* fx
   if (n == 0) return hipSuccess;
   pack_records_kernel<<<dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s>>>(st, mass, fx, fy, fz, pos, base, n,
                                                                                out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_positions(const SoA& st, const uint32_t* pos, uint32_t base, uint32_t n, float* out,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  pack_positions_kernel<<<dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s>>>(st, pos, base, n, out);
   return hipGetLastError();
 }
 

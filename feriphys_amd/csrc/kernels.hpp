@@ -164,6 +164,9 @@ hipError_t launch_fill(float* p, uint64_t n, float v, hipStream_t s);
 // i from device slot base + pos[i] (pos == nullptr: base + i).
 hipError_t launch_pack_records(const SoA& st, const float* mass, const float* fx, const float* fy, const float* fz,
                                const uint32_t* pos, uint32_t base, uint32_t n, float* out, hipStream_t s);
+// The positions of device vertices [base, base + n), 3 floats each, in the reference's order.
+hipError_t launch_pack_positions(const SoA& st, const uint32_t* pos, uint32_t base, uint32_t n, float* out,
+                                 hipStream_t s);
 // The inverse: 10 n floats of records (reference order) into the state, masses and forces.
 hipError_t launch_unpack_records(const float* in, const uint32_t* pos, uint32_t base, uint32_t n, const SoA& st,
                                  float* mass, float* fx, float* fy, float* fz, hipStream_t s);

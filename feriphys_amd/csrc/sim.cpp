@@ -1230,10 +1230,17 @@ int fp_sim_set_points(fp_sim* s, uint64_t mesh, const float* rec, uint64_t n) {
 int fp_sim_get_vertices(fp_sim* s, uint64_t mesh, float* pos, uint64_t* idx) {
   if (int e = check_sim_mesh(s, mesh)) return e;
   const MeshSlot& m = s->meshes[mesh];
-  std::vector<float> rec(10 * m.host.n);
-  if (m.host.n)
-    if (int e = fp_sim_get_points(s, mesh, rec.data())) return e;
-  copy_vertices(m.host, rec.data(), pos, idx);
+  const uint64_t n = m.host.n;
+  if (pos && n) {  // positions packed in the reference's order on the device, one copy
+    HIP_TRY(s->rec_stage.reserve(3 * n));
+    const auto& st = s->state[s->cur];
+    const fp::SoA cur{st[0].p, st[1].p, st[2].p, st[3].p, st[4].p, st[5].p};
+    HIP_TRY(fp::launch_pack_positions(cur, m.pos.empty() ? nullptr : s->d_pos.p + m.vbase, (uint32_t)m.vbase,
+                                      (uint32_t)n, s->rec_stage.p, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    HIP_TRY(hipMemcpy(pos, s->rec_stage.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost));
+  }
+  if (idx) std::copy(m.host.f_v.begin(), m.host.f_v.end(), idx);
   return FP_OK;
 }
 

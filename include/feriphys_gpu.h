@@ -175,7 +175,9 @@ int fp_sim_count_nonfinite(fp_sim* sim, uint64_t* counts);
 int fp_sim_get_points(fp_sim* sim, uint64_t mesh, float* records);
 /* Checkpoint restore of one mesh's records (same layout; n must equal its point count). */
 int fp_sim_set_points(fp_sim* sim, uint64_t mesh, const float* records, uint64_t n);
-/* get_meshes()[mesh].get_vertices() (springy_mesh.rs:500-513). */
+/* get_meshes()[mesh].get_vertices() (springy_mesh.rs:500-513): 3 floats per vertex and the 3 vertex
+ * indices of every face.  Either output may be NULL; a renderer that keeps the (static) index buffer
+ * passes NULL for it and gets the positions in one 12-byte-per-vertex copy. */
 int fp_sim_get_vertices(fp_sim* sim, uint64_t mesh, float* positions, uint64_t* indices);
 /* SpringyMesh::accumulate_forces (springy_mesh.rs:515-525) with the mesh's config: gravity,
  * struts, faces, pinned F = 0.  The forces appear in fp_sim_get_points records until cleared. */

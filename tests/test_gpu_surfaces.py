@@ -240,3 +240,24 @@ def test_strut_classes_choose_the_kernel_and_match_oracle(n_over):
     g.step(120)
     o.step(120)
     assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
+
+
+def test_get_vertices_positions_and_faces():
+    """SpringyMesh::get_vertices through the simulation: the positions equal the records' (in the
+    reference's vertex order, packed on the device) and the face indices the oracle's; a NULL index
+    buffer returns the positions alone."""
+    import ctypes as C
+
+    sc = wl.config4(n_meshes=2, seed=29, levels=[3, 2])
+    g, o = wl.build_surface_gpu(sc), orc.build_surface_scene(sc)
+    g.step(30)
+    o.step(30)
+    for k in range(2):
+        pos, idx = g.get_vertices(k)
+        rec = g.get_points(k)
+        assert (bits(pos) == bits(rec[:, 1:4])).all()
+        assert (bits(pos) == bits(o.mesh(k).points()[:, 1:4])).all()
+        assert (idx == o.mesh(k).faces()[1].reshape(-1)).all()
+        only = np.zeros_like(pos)
+        rc = fa.lib().fp_sim_get_vertices(g.handle, k, only.ctypes.data_as(C.POINTER(C.c_float)), None)
+        assert rc == 0 and (bits(only) == bits(pos)).all()
