@@ -40,6 +40,19 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.SIGNATURES) == set(syms)
 
 
+def test_c99_caller(tmp_path):
+    """The boundary from plain C: tests/native/abi_c_check.c includes the header as C99, links the
+    library and checks construction, readback, error codes and ownership (and a step when a GPU is
+    present)."""
+    exe = str(tmp_path / "abi_c_check")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "native", "abi_c_check.c"), "-L", libdir, "-lferiphys_gpu",
+                    "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip()
+    assert out == "ok", out
+
+
 def test_rust_binding_declares_every_entry_point():
     """The appendix of INTEGRATION.md (the extern "C" block a Rust caller adds) declares each
     function of the header once, with the header's parameter count."""
