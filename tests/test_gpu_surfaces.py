@@ -289,3 +289,26 @@ def test_icospheres_euler_and_per_vertex_masses_match_oracle(kernel):
         a, b = g.get_points(k), o.mesh(k).points()
         assert np.isfinite(a).all()
         assert (bits(a) == bits(b)).all(), k
+
+
+@pytest.mark.parametrize("kernel", [fa.FP_KERNEL_AUTO, fa.FP_KERNEL_CSR])
+def test_icosphere_onto_sphere_obstacle_matches_oracle(kernel):
+    """A general mesh dropped onto a curved obstacle (a 320-face icosphere, the broad-phase grids'
+    path) beside the ground plane: first-hit faces and responses bit-identical to the oracle over
+    240 steps, the mesh draped over the sphere's cap."""
+    sc = wl.config4(n_meshes=2, seed=37, levels=[3, 3])
+    for m in sc.meshes:
+        m.positions[:, 1] -= m.positions[:, 1].min() - 0.02
+    c0 = sc.meshes[0].positions.astype(np.float64).mean(axis=0)
+    sc.obstacles = list(sc.obstacles) + [wl.sphere_obstacle(0.2, (c0[0] + 0.01, -0.19, c0[2] + 0.005), 2)]
+    g, o = wl.build_surface_gpu(sc), orc.build_surface_scene(sc)
+    g.set_kernel(kernel, fa.FP_MATH_EXACT)
+    g.step(240)
+    o.step(240)
+    for k in range(2):
+        a, b = g.get_points(k), o.mesh(k).points()
+        assert np.isfinite(a).all()
+        assert (bits(a) == bits(b)).all(), k
+    # the first mesh (soft) drapes over the sphere's cap, whose top is ~9.5 mm above the ground;
+    # the second lies flat on the ground
+    assert o.mesh(0).points()[:, 2].max() > 0.009 > 0.008 > o.mesh(1).points()[:, 2].max()
