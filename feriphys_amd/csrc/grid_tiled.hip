@@ -32,10 +32,14 @@ __device__ __forceinline__ StrutK strut_k(float dX, float dY, float kn, float cn
 
 // The LDS of one TR x TC tile: state tile + halo (rows r0-2 .. r0+TR+1, cols c0-2 .. c0+TC+1), the
 // drag + lift of the faces of quads (r', c') with r' in [r0-1, r0+TR), c' in [c0-1, c0+TC), and the
-// strut constant tables.
+// strut constant tables.  A staged row holds its six fields one after another (field q of column c
+// at q * (TC + 4) + c) and rows are kStRow words apart, kStRow = 16 (mod 64): the four rows of
+// sixteen columns a wavefront reads in the per-vertex pass, shifted by any neighbour offset, then
+// fall on distinct LDS banks.
 template <int TR, int TC>
 struct TileLds {
-  float st[6][TR + 4][TC + 4];
+  static constexpr int kStRow = (6 * (TC + 4) - 16 + 63) / 64 * 64 + 16;
+  float st[TR + 4][kStRow];
   float face[2][TR + 1][TC + 1][3];
   StrutK shear[TR + 1][TC + 1];
   StrutK h[TC + 1], v[TR + 1], bh[TC + 2], bv[TR + 2];
@@ -71,7 +75,7 @@ __device__ __forceinline__ void tile_step(TileLds<TR, TC>& L, const StepArgs& A,
     if (r < 0 || r >= R || c < 0 || c >= C) continue;
     const uint32_t g = G.base + (uint32_t)(r * C + c);
 #pragma unroll
-    for (int q = 0; q < 6; ++q) s_st[q][lr][lc] = src[q][g];
+    for (int q = 0; q < 6; ++q) s_st[lr][q * TW + lc] = src[q][g];
   }
   // strut constant tables (construction grid only: no state needed)
   for (int j = tid; j < TC + 1; j += kBlock) {  // (c0-1+j, c0+j)
@@ -97,10 +101,14 @@ __device__ __forceinline__ void tile_step(TileLds<TR, TC>& L, const StepArgs& A,
   }
   __syncthreads();
 
-  auto Pl = [&](int r, int c) { return f3{s_st[0][r - r0 + 2][c - c0 + 2], s_st[1][r - r0 + 2][c - c0 + 2],
-                                          s_st[2][r - r0 + 2][c - c0 + 2]}; };
-  auto Vl = [&](int r, int c) { return f3{s_st[3][r - r0 + 2][c - c0 + 2], s_st[4][r - r0 + 2][c - c0 + 2],
-                                          s_st[5][r - r0 + 2][c - c0 + 2]}; };
+  auto Pl = [&](int r, int c) {
+    const float* row = s_st[r - r0 + 2] + (c - c0 + 2);
+    return f3{row[0], row[TW], row[2 * TW]};
+  };
+  auto Vl = [&](int r, int c) {
+    const float* row = s_st[r - r0 + 2] + (c - c0 + 2);
+    return f3{row[3 * TW], row[4 * TW], row[5 * TW]};
+  };
 
   // ---- phase 1: drag + lift of every face that touches the tile, once per face
   // A(q) = [(r',c'+1), (r'+1,c'), (r',c')], B(q) = [(r',c'+1), (r'+1,c'+1), (r'+1,c')]
