@@ -53,6 +53,19 @@ def test_c99_caller(tmp_path):
     assert out == "ok", out
 
 
+def test_cpp_mirror_reads_like_the_reference_tests(tmp_path):
+    """include/feriphys.hpp mirrors the reference's springy API in C++ over the C ABI;
+    tests/native/mirror_check.cpp restates the reference's SpringyMesh unit tests (ctor_triangle,
+    strip_ctor, get_vertices) through it, plus Cloth::new's counts and the panics as exceptions."""
+    exe = str(tmp_path / "mirror_check")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "native", "mirror_check.cpp"), "-L", libdir, "-lferiphys_gpu",
+                    "-Wl,-rpath," + libdir, "-o", exe], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip()
+    assert out == "ok", out
+
+
 def test_rust_binding_declares_every_entry_point():
     """The appendix of INTEGRATION.md (the extern "C" block a Rust caller adds) declares each
     function of the header once, with the header's parameter count."""
