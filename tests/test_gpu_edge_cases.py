@@ -86,3 +86,45 @@ def test_nonfinite_counts_per_mesh():
     c = sim.count_nonfinite()
     announce(f"non-finite vertices after 3 steps: {list(c)}")
     assert c[0] == 0 and c[1] > 1
+
+
+@pytest.mark.parametrize("kernel", [fa.FP_KERNEL_AUTO, fa.FP_KERNEL_CSR])
+def test_strut_chain_without_faces(kernel):
+    """A mesh with no triangles, only add_strut struts: a chain of 40 vertices hanging from a pin
+    and falling onto the floor (no face forces, so the patch kernel runs with no face LDS)."""
+    n = 40
+    pos = np.stack([np.linspace(0.0, 0.39, n), np.full(n, 0.05), np.zeros(n)], 1).astype(np.float32)
+    gm = fa.SpringyMesh(pos, np.zeros(0, np.uint64), 0.4, 1.0, 1.0)
+    om = orc.Mesh.new(pos, np.zeros(0, np.uint64), 0.4, 1.0, 1.0)
+    for i in range(n - 1):
+        gm.add_strut((i, i + 1), 5.0, 0.01)
+        om.add_strut(i, i + 1, 5.0, 0.01)
+    gm.add_pin(0)
+    om.add_pin(0)
+    gc, oc = cfg_pair()
+    g = fa.Simulation([gm], [fa.CollidableMesh(*FLOOR)])
+    g.config = gc
+    g.set_kernel(kernel, fa.FP_MATH_EXACT)
+    assert g.get_mesh_kernel(0) == (fa.FP_STEP_PATCH if kernel == fa.FP_KERNEL_AUTO else fa.FP_STEP_CSR)
+    o = orc.Simulation([om], [orc.Obstacle(*FLOOR)])
+    o.set_config(oc)
+    g.step(300)
+    o.step(300)
+    a, b = g.get_points(0), o.mesh(0).points()
+    assert np.isfinite(a).all() and b[:, 2].min() > -1e-3
+    assert (bits(a) == bits(b)).all()
+
+
+def test_zero_timestep_leaves_the_state():
+    """dt = 0 (a valid Duration): every step returns Duration 0 and the records stay as the oracle's
+    (x + 0 v, v + 0 a), bit for bit."""
+    pos = np.array([[0, 0.3, 0], [0.1, 0.3, 0], [0, 0.3, 0.1]], np.float32)
+    idx = np.array([0, 1, 2], np.uint64)
+    gc, oc = cfg_pair(dt=0.0)
+    g = fa.Simulation([fa.SpringyMesh(pos, idx, 0.03, 20.0, 0.1)], [fa.CollidableMesh(*FLOOR)])
+    g.config = gc
+    o = orc.Simulation([orc.Mesh.new(pos, idx, 0.03, 20.0, 0.1)], [orc.Obstacle(*FLOOR)])
+    o.set_config(oc)
+    assert g.step(3) == 0.0
+    o.step(3)
+    assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
