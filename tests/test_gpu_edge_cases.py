@@ -1,6 +1,7 @@
 """Degenerate and boundary inputs on the GPU, each against the oracle: a simulation with no meshes,
 a general mesh with vertices that no triangle uses (patches of one vertex), pinned vertices on the
-patch kernel, a single triangle, and a rigid-body simulation with no bodies."""
+patch kernel, a single triangle, a mesh of struts without faces, a zero timestep, per-mesh counts of
+non-finite vertices, and a rigid-body simulation with no bodies."""
 import numpy as np
 import pytest
 
