@@ -339,6 +339,261 @@ int size_class(const fp::ClothDesc& d) {
   return m < 16 ? 0 : m < 32 ? 1 : m < 64 ? 2 : 3;
 }
 
+// Uploads the incidence lists of the packed meshes (vertex-centric: incident struts in strut
+// order, incident faces in face order) in the sliced layout of fp::CsrView, the strut classes of
+// the patch kernel, and the patches of the general meshes.
+int build_incidence(fp_sim* s) {
+  const uint64_t N = s->N, F = s->F;
+  std::vector<uint32_t> scnt(N, 0), fcnt(N, 0), sfirst(N), ffirst(N), fv(3 * F);
+  for (auto& ms_ : s->meshes) {
+    const auto& h = ms_.host;
+    for (uint64_t k = 0; k < h.n_struts(); ++k) {
+      scnt[ms_.dev(h.s_v0[k])]++;
+      scnt[ms_.dev(h.s_v1[k])]++;
+    }
+    for (uint64_t k = 0; k < h.f_v.size(); ++k) fcnt[ms_.dev(h.f_v[k])]++;
+  }
+  // slices: kSlice consecutive device ids from each block start (the wavefronts of the block's
+  // workgroup); a slice takes kSlice x (its largest count) entries, lane l's entry j at the slice
+  // base + kSlice j + l
+  auto slices = [&](const std::vector<uint32_t>& cnt, std::vector<uint32_t>& first) -> uint64_t {
+    uint64_t total = 0;
+    for (auto& ms_ : s->meshes)
+      for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b)
+        for (uint64_t g0 = ms_.bstart[b]; g0 < ms_.bstart[b + 1]; g0 += fp::kSlice) {
+          const uint64_t g1 = std::min<uint64_t>(ms_.bstart[b + 1], g0 + fp::kSlice);
+          uint32_t mx = 0;
+          for (uint64_t d = g0; d < g1; ++d) mx = std::max(mx, cnt[ms_.vbase + d]);
+          for (uint64_t d = g0; d < g1; ++d) first[ms_.vbase + d] = (uint32_t)(total + (d - g0));
+          total += (uint64_t)fp::kSlice * mx;
+        }
+    return total;
+  };
+  const uint64_t n_sent = slices(scnt, sfirst), n_fent = slices(fcnt, ffirst);
+  if (n_sent >= (1ull << 32) || n_fent >= (1ull << 32)) return fail(FP_EINVAL, "simulation too large for 32-bit ids");
+  std::vector<fp::StrutEnt> sent(n_sent + 1, fp::StrutEnt{0, 1.0f, 0.0f, 0.0f});
+  std::vector<uint32_t> fslot(3 * F), sfill(N, 0), ffill(N, 0);
+  std::vector<uint32_t> frank(3 * F);  // per face corner (reference face order): the face's rank among its vertex's faces
+  // the patch kernel's strut classes: per mesh the distinct nominal (stiffness, damping) pairs in
+  // first-appearance order, each strut's constants checked to be (1.0 / len) * its pair's
+  std::vector<uint16_t> scls(n_sent + 1, 0);
+  std::vector<float2> cls_tab(s->meshes.size() * fp::kPatchClasses, float2{0.0f, 0.0f});
+  std::vector<uint8_t> cls_ok(s->meshes.size(), 1);
+  for (size_t mi = 0; mi < s->meshes.size(); ++mi) {
+    auto& ms_ = s->meshes[mi];
+    const auto& h = ms_.host;
+    uint32_t n_cls = 0;
+    for (uint64_t k = 0; k < h.n_struts(); ++k) {
+      uint32_t a = (uint32_t)ms_.dev(h.s_v0[k]), b = (uint32_t)ms_.dev(h.s_v1[k]);
+      const uint32_t ea = sfirst[a] + fp::kSlice * sfill[a]++, eb = sfirst[b] + fp::kSlice * sfill[b]++;
+      sent[ea] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
+      sent[eb] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
+      if (!cls_ok[mi]) continue;
+      const float kn = h.s_kn[k], cn = h.s_cn[k], inv = 1.0f / h.s_len[k];
+      if (!same_bits(inv * kn, h.s_k[k]) || !same_bits(inv * cn, h.s_c[k])) { cls_ok[mi] = 0; continue; }
+      float2* tab = cls_tab.data() + mi * fp::kPatchClasses;
+      uint32_t c = 0;
+      while (c < n_cls && !(same_bits(tab[c].x, kn) && same_bits(tab[c].y, cn))) ++c;
+      if (c == n_cls) {
+        if (n_cls == fp::kPatchClasses) { cls_ok[mi] = 0; continue; }
+        tab[n_cls++] = float2{kn, cn};
+      }
+      scls[ea] = scls[eb] = (uint16_t)c;
+    }
+    // faces are stored (for the face kernel) in ascending order of their lowest device vertex id,
+    // so that neighbouring threads gather neighbouring vertices and write neighbouring entries;
+    // each vertex's entries list its faces in face-index order
+    std::vector<uint32_t> slot3(3 * h.n_faces()), lo(h.n_faces()), at(h.n_faces());
+    for (uint64_t t = 0; t < h.n_faces(); ++t) {
+      uint32_t m = 0xffffffffu;
+      for (uint32_t j = 0; j < 3; ++j) {
+        uint32_t v = (uint32_t)ms_.dev(h.f_v[3 * t + j]);
+        frank[ms_.fbase * 3 + 3 * t + j] = ffill[v];
+        slot3[3 * t + j] = ffirst[v] + fp::kSlice * ffill[v]++;
+        m = std::min(m, v);
+      }
+      lo[t] = m;
+      at[t] = (uint32_t)t;
+    }
+    std::stable_sort(at.begin(), at.end(), [&](uint32_t a, uint32_t b) { return lo[a] < lo[b]; });
+    for (uint64_t q = 0; q < h.n_faces(); ++q)
+      for (uint32_t j = 0; j < 3; ++j) {
+        fv[3 * (ms_.fbase + q) + j] = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)at[q] + j]);
+        fslot[3 * (ms_.fbase + q) + j] = slot3[3 * (uint64_t)at[q] + j];
+      }
+  }
+  // patches of the general meshes (block b of every mesh is h_patches[b]): halo, faces, LDS
+  // indices, each patch vertex's corners.  A mesh with a patch whose vertices and halo exceed
+  // kPatchLocal, or whose faces exceed kPatchFaceCap, stays on the CSR kernel.
+  std::vector<uint16_t> sloc(n_sent + 1, 0);
+  std::vector<uint32_t> halo;
+  std::vector<fp::PatchFace> pfaces;
+  std::vector<fp::PatchCorner> pcorner(n_fent + 1, fp::PatchCorner{0, 0, 0, 0});
+  std::vector<int32_t> mark(N, -1);
+  for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
+    auto& ms_ = s->meshes[mi];
+    const auto& h = ms_.host;
+    ms_.patch_ok = !ms_.perm.empty() && cls_ok[mi];
+    std::vector<uint32_t> vf_row, vf;  // reference faces of each device-local vertex
+    if (ms_.patch_ok) {
+      vf_row.assign(h.n + 1, 0);
+      for (uint64_t k = 0; k < h.f_v.size(); ++k) vf_row[ms_.pos[h.f_v[k]] + 1]++;
+      for (uint64_t d = 0; d < h.n; ++d) vf_row[d + 1] += vf_row[d];
+      vf.resize(h.f_v.size());
+      std::vector<uint32_t> fill(vf_row.begin(), vf_row.end() - 1);
+      for (uint64_t k = 0; k < h.f_v.size(); ++k) vf[fill[ms_.pos[h.f_v[k]]]++] = (uint32_t)(k / 3);
+    }
+    std::vector<uint8_t> fseen(ms_.patch_ok ? h.n_faces() : 0, 0);
+    for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b) {
+      const uint32_t b0 = (uint32_t)ms_.vbase + ms_.bstart[b], b1 = (uint32_t)ms_.vbase + ms_.bstart[b + 1];
+      fp::PatchRange pr{mi, b0, b1, (uint32_t)halo.size(), 0u, (uint32_t)pfaces.size(), 0u, 0u};
+      if (ms_.patch_ok) {
+        const size_t h0 = halo.size(), q0 = pfaces.size();
+        auto local = [&](uint32_t g) -> uint32_t {  // LDS index of device vertex g (adds it to the halo)
+          if (g >= b0 && g < b1) return g - b0;
+          if (mark[g] < 0) { mark[g] = (int32_t)(halo.size() - h0); halo.push_back(g); }
+          return (b1 - b0) + (uint32_t)mark[g];
+        };
+        for (uint32_t i = b0; i < b1; ++i)
+          for (uint32_t j = 0; j < scnt[i]; ++j) {
+            const uint32_t e = sfirst[i] + fp::kSlice * j;
+            sloc[e] = (uint16_t)std::min<uint32_t>(local(sent[e].nbr), fp::kPatchLocal - 1);
+          }
+        std::vector<uint32_t> fl;
+        for (uint32_t i = b0; i < b1; ++i)
+          for (uint32_t k = vf_row[i - ms_.vbase]; k < vf_row[i - ms_.vbase + 1]; ++k)
+            if (!fseen[vf[k]]) { fseen[vf[k]] = 1; fl.push_back(vf[k]); }
+        for (uint32_t qi = 0; qi < (uint32_t)fl.size(); ++qi) {
+          const uint32_t f = fl[qi];
+          fp::PatchFace pf{};
+          uint32_t g3[3];
+          for (int c = 0; c < 3; ++c) {
+            g3[c] = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)f + c]);
+            pf.v[c] = (uint16_t)std::min<uint32_t>(local(g3[c]), fp::kPatchLocal - 1);
+          }
+          // corner c of a patch vertex: Face::vertex_angle's operands are the next and the
+          // previous corner (springy_mesh.rs:163-172), in the entry of the face's rank among
+          // that vertex's faces
+          for (int c = 0; c < 3; ++c)
+            if (g3[c] >= b0 && g3[c] < b1)
+              pcorner[ffirst[g3[c]] + fp::kSlice * frank[3 * (ms_.fbase + f) + c]] =
+                  fp::PatchCorner{(uint16_t)std::min<uint32_t>(qi, fp::kPatchFaceCap - 1), pf.v[(c + 1) % 3],
+                                  pf.v[(c + 2) % 3], 0};
+          pfaces.push_back(pf);
+        }
+        for (uint32_t f : fl) fseen[f] = 0;
+        pr.n_halo = (uint32_t)(halo.size() - h0);
+        pr.n_faces = (uint32_t)(pfaces.size() - q0);
+        if ((b1 - b0) + pr.n_halo > fp::kPatchLocal || pr.n_faces > fp::kPatchFaceCap) ms_.patch_ok = false;
+        for (size_t k = h0; k < halo.size(); ++k) mark[halo[k]] = -1;
+      }
+      s->h_patches.push_back(pr);
+    }
+  }
+  pfaces.push_back(fp::PatchFace{});
+  HIP_TRY(s->patch_faces.upload(pfaces.data(), pfaces.size()));
+  HIP_TRY(s->p_corner.upload(pcorner.data(), pcorner.size()));
+  {
+    std::vector<fp::PatchStrut> pent(n_sent + 1);
+    for (uint64_t e = 0; e <= n_sent; ++e) pent[e] = fp::PatchStrut{sent[e].len, sloc[e], scls[e]};
+    HIP_TRY(s->p_ent.upload(pent.data(), pent.size()));
+    HIP_TRY(s->p_cls.upload(cls_tab.data(), cls_tab.size()));
+  }
+  halo.push_back(0);
+  HIP_TRY(s->halo.upload(halo.data(), halo.size()));
+  HIP_TRY(s->s_first.upload(sfirst.data(), sfirst.size()));
+  HIP_TRY(s->s_cnt.upload(scnt.data(), scnt.size()));
+  HIP_TRY(s->s_ent.upload(sent.data(), sent.size()));
+  HIP_TRY(s->face_c.alloc(n_fent + 1));
+  HIP_TRY(s->f_first.upload(ffirst.data(), ffirst.size()));
+  HIP_TRY(s->f_cnt.upload(fcnt.data(), fcnt.size()));
+  HIP_TRY(s->f_slot.upload(fslot.data(), fslot.size()));
+  HIP_TRY(s->f_v.upload(fv.data(), fv.size()));
+  return FP_OK;
+}
+
+// Uploads the torsional hinges (springy_mesh.rs:348-393 builds them; :555-567 names the four
+// vertices) and each vertex's hinge entries.
+int build_hinges(fp_sim* s) {
+  const uint64_t N = s->N;
+  std::vector<fp::HingeEnt> hs;
+  for (auto& ms_ : s->meshes) {
+    const auto& h = ms_.host;
+    ms_.hinge_begin = (uint32_t)hs.size();
+    for (uint64_t k = 0; k < h.n_struts(); ++k) {
+      if (!h.t_has[k] || h.s_f0[k] < 0 || h.s_f1[k] < 0) continue;
+      const uint32_t a = h.s_v0[k], b = h.s_v1[k];
+      auto other = [&](int64_t f) -> uint32_t {  // utils::tuple_difference(face vertices, strut)
+        for (int j = 0; j < 3; ++j) {
+          const uint32_t v = (uint32_t)h.f_v[3 * (size_t)f + j];
+          if (v != a && v != b) return v;
+        }
+        return a;
+      };
+      fp::HingeEnt e{};
+      e.v[0] = (uint32_t)ms_.dev(a);
+      e.v[1] = (uint32_t)ms_.dev(b);
+      e.v[2] = (uint32_t)ms_.dev(other(h.s_f0[k]));
+      e.v[3] = (uint32_t)ms_.dev(other(h.s_f1[k]));
+      e.k = h.t_k[k];
+      e.c = h.t_c[k];
+      e.rest = h.t_rest[k];
+      hs.push_back(e);
+    }
+    ms_.hinge_end = (uint32_t)hs.size();
+  }
+  s->n_hinges = (uint32_t)hs.size();
+  if (!hs.empty()) {
+    std::vector<uint32_t> trow(N + 1, 0);
+    for (const auto& e : hs)
+      for (int r = 0; r < 4; ++r) trow[e.v[r] + 1]++;
+    for (uint64_t i = 0; i < N; ++i) trow[i + 1] += trow[i];
+    std::vector<uint32_t> tent(trow[N]), tfill(trow.begin(), trow.end() - 1);
+    for (uint32_t k = 0; k < hs.size(); ++k)
+      for (uint32_t r = 0; r < 4; ++r) tent[tfill[hs[k].v[r]]++] = (k << 2) | r;
+    HIP_TRY(s->hinges.upload(hs.data(), hs.size()));
+    HIP_TRY(s->t_row.upload(trow.data(), trow.size()));
+    HIP_TRY(s->t_ent.upload(tent.data(), tent.size()));
+    HIP_TRY(s->hinge_f.alloc(4 * hs.size()));
+  }
+  return FP_OK;
+}
+
+// Uploads the cloth descriptors of the grid kernels and the vertex and face block tables.
+int build_cloth_views(fp_sim* s) {
+  std::vector<fp::ClothView> cv;
+  for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
+    MeshSlot& ms_ = s->meshes[mi];
+    const fp::ClothDesc& d = ms_.host.cloth;
+    fp::ClothView c{};
+    c.base = (uint32_t)ms_.vbase;
+    c.rows = d.rows;
+    c.cols = d.cols;
+    c.spacing = d.spacing;
+    c.cx = (float)d.cols / 2.0f * d.spacing;
+    c.cy = (float)d.rows / 2.0f * d.spacing;
+    c.p0 = d.position[0];
+    c.p1 = d.plane == FP_PLANE_XZ ? d.position[2] : d.position[1];
+    c.tk = d.tk; c.tc = d.tc; c.sk = d.sk; c.sc = d.sc; c.bk = d.bk; c.bc = d.bc;
+    cv.push_back(c);
+    ms_.blk_begin = (uint32_t)s->h_blocks.size();
+    for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b)
+      s->h_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.vbase + ms_.bstart[b]),
+                                           (uint32_t)(ms_.vbase + ms_.bstart[b + 1]), 0u});
+    ms_.blk_end = (uint32_t)s->h_blocks.size();
+    ms_.fblk_begin = (uint32_t)s->h_face_blocks.size();
+    const uint64_t nf = ms_.host.n_faces();
+    for (uint64_t b = 0; b < nf; b += fp::kBlock)
+      s->h_face_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.fbase + b),
+                                                (uint32_t)(ms_.fbase + std::min<uint64_t>(nf, b + fp::kBlock)), 0u});
+    ms_.fblk_end = (uint32_t)s->h_face_blocks.size();
+  }
+  HIP_TRY(s->cloths.upload(cv.data(), cv.size()));
+  HIP_TRY(s->all_blocks.upload(s->h_blocks.data(), s->h_blocks.size()));
+  HIP_TRY(s->all_face_blocks.upload(s->h_face_blocks.data(), s->h_face_blocks.size()));
+  return FP_OK;
+}
+
 }  // namespace fpsim
 
 extern "C" {
@@ -609,253 +864,9 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
     HIP_TRY(s->pins.upload(bits.data(), bits.size()));
   }
 
-  // incidence lists (vertex-centric, incident struts in strut order, incident faces in face
-  // order) in the sliced layout of fp::CsrView
-  {
-    std::vector<uint32_t> scnt(N, 0), fcnt(N, 0), sfirst(N), ffirst(N), fv(3 * F);
-    for (auto& ms_ : s->meshes) {
-      const auto& h = ms_.host;
-      for (uint64_t k = 0; k < h.n_struts(); ++k) {
-        scnt[ms_.dev(h.s_v0[k])]++;
-        scnt[ms_.dev(h.s_v1[k])]++;
-      }
-      for (uint64_t k = 0; k < h.f_v.size(); ++k) fcnt[ms_.dev(h.f_v[k])]++;
-    }
-    // slices: kSlice consecutive device ids from each block start (the wavefronts of the block's
-    // workgroup); a slice takes kSlice x (its largest count) entries, lane l's entry j at the slice
-    // base + kSlice j + l
-    auto slices = [&](const std::vector<uint32_t>& cnt, std::vector<uint32_t>& first) -> uint64_t {
-      uint64_t total = 0;
-      for (auto& ms_ : s->meshes)
-        for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b)
-          for (uint64_t g0 = ms_.bstart[b]; g0 < ms_.bstart[b + 1]; g0 += fp::kSlice) {
-            const uint64_t g1 = std::min<uint64_t>(ms_.bstart[b + 1], g0 + fp::kSlice);
-            uint32_t mx = 0;
-            for (uint64_t d = g0; d < g1; ++d) mx = std::max(mx, cnt[ms_.vbase + d]);
-            for (uint64_t d = g0; d < g1; ++d) first[ms_.vbase + d] = (uint32_t)(total + (d - g0));
-            total += (uint64_t)fp::kSlice * mx;
-          }
-      return total;
-    };
-    const uint64_t n_sent = slices(scnt, sfirst), n_fent = slices(fcnt, ffirst);
-    if (n_sent >= (1ull << 32) || n_fent >= (1ull << 32)) return fail(FP_EINVAL, "simulation too large for 32-bit ids");
-    std::vector<fp::StrutEnt> sent(n_sent + 1, fp::StrutEnt{0, 1.0f, 0.0f, 0.0f});
-    std::vector<uint32_t> fslot(3 * F), sfill(N, 0), ffill(N, 0);
-    std::vector<uint32_t> frank(3 * F);  // per face corner (reference face order): the face's rank among its vertex's faces
-    // the patch kernel's strut classes: per mesh the distinct nominal (stiffness, damping) pairs in
-    // first-appearance order, each strut's constants checked to be (1.0 / len) * its pair's
-    std::vector<uint16_t> scls(n_sent + 1, 0);
-    std::vector<float2> cls_tab(s->meshes.size() * fp::kPatchClasses, float2{0.0f, 0.0f});
-    std::vector<uint8_t> cls_ok(s->meshes.size(), 1);
-    for (size_t mi = 0; mi < s->meshes.size(); ++mi) {
-      auto& ms_ = s->meshes[mi];
-      const auto& h = ms_.host;
-      uint32_t n_cls = 0;
-      for (uint64_t k = 0; k < h.n_struts(); ++k) {
-        uint32_t a = (uint32_t)ms_.dev(h.s_v0[k]), b = (uint32_t)ms_.dev(h.s_v1[k]);
-        const uint32_t ea = sfirst[a] + fp::kSlice * sfill[a]++, eb = sfirst[b] + fp::kSlice * sfill[b]++;
-        sent[ea] = fp::StrutEnt{b, h.s_len[k], h.s_k[k], h.s_c[k]};
-        sent[eb] = fp::StrutEnt{a, h.s_len[k], h.s_k[k], h.s_c[k]};
-        if (!cls_ok[mi]) continue;
-        const float kn = h.s_kn[k], cn = h.s_cn[k], inv = 1.0f / h.s_len[k];
-        if (!same_bits(inv * kn, h.s_k[k]) || !same_bits(inv * cn, h.s_c[k])) { cls_ok[mi] = 0; continue; }
-        float2* tab = cls_tab.data() + mi * fp::kPatchClasses;
-        uint32_t c = 0;
-        while (c < n_cls && !(same_bits(tab[c].x, kn) && same_bits(tab[c].y, cn))) ++c;
-        if (c == n_cls) {
-          if (n_cls == fp::kPatchClasses) { cls_ok[mi] = 0; continue; }
-          tab[n_cls++] = float2{kn, cn};
-        }
-        scls[ea] = scls[eb] = (uint16_t)c;
-      }
-      // faces are stored (for the face kernel) in ascending order of their lowest device vertex id,
-      // so that neighbouring threads gather neighbouring vertices and write neighbouring entries;
-      // each vertex's entries list its faces in face-index order
-      std::vector<uint32_t> slot3(3 * h.n_faces()), lo(h.n_faces()), at(h.n_faces());
-      for (uint64_t t = 0; t < h.n_faces(); ++t) {
-        uint32_t m = 0xffffffffu;
-        for (uint32_t j = 0; j < 3; ++j) {
-          uint32_t v = (uint32_t)ms_.dev(h.f_v[3 * t + j]);
-          frank[ms_.fbase * 3 + 3 * t + j] = ffill[v];
-          slot3[3 * t + j] = ffirst[v] + fp::kSlice * ffill[v]++;
-          m = std::min(m, v);
-        }
-        lo[t] = m;
-        at[t] = (uint32_t)t;
-      }
-      std::stable_sort(at.begin(), at.end(), [&](uint32_t a, uint32_t b) { return lo[a] < lo[b]; });
-      for (uint64_t q = 0; q < h.n_faces(); ++q)
-        for (uint32_t j = 0; j < 3; ++j) {
-          fv[3 * (ms_.fbase + q) + j] = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)at[q] + j]);
-          fslot[3 * (ms_.fbase + q) + j] = slot3[3 * (uint64_t)at[q] + j];
-        }
-    }
-    // patches of the general meshes (block b of every mesh is h_patches[b]): halo, faces, LDS
-    // indices, each patch vertex's corners.  A mesh with a patch whose vertices and halo exceed
-    // kPatchLocal, or whose faces exceed kPatchFaceCap, stays on the CSR kernel.
-    std::vector<uint16_t> sloc(n_sent + 1, 0);
-    std::vector<uint32_t> halo;
-    std::vector<fp::PatchFace> pfaces;
-    std::vector<fp::PatchCorner> pcorner(n_fent + 1, fp::PatchCorner{0, 0, 0, 0});
-    std::vector<int32_t> mark(N, -1);
-    for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
-      auto& ms_ = s->meshes[mi];
-      const auto& h = ms_.host;
-      ms_.patch_ok = !ms_.perm.empty() && cls_ok[mi];
-      std::vector<uint32_t> vf_row, vf;  // reference faces of each device-local vertex
-      if (ms_.patch_ok) {
-        vf_row.assign(h.n + 1, 0);
-        for (uint64_t k = 0; k < h.f_v.size(); ++k) vf_row[ms_.pos[h.f_v[k]] + 1]++;
-        for (uint64_t d = 0; d < h.n; ++d) vf_row[d + 1] += vf_row[d];
-        vf.resize(h.f_v.size());
-        std::vector<uint32_t> fill(vf_row.begin(), vf_row.end() - 1);
-        for (uint64_t k = 0; k < h.f_v.size(); ++k) vf[fill[ms_.pos[h.f_v[k]]]++] = (uint32_t)(k / 3);
-      }
-      std::vector<uint8_t> fseen(ms_.patch_ok ? h.n_faces() : 0, 0);
-      for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b) {
-        const uint32_t b0 = (uint32_t)ms_.vbase + ms_.bstart[b], b1 = (uint32_t)ms_.vbase + ms_.bstart[b + 1];
-        fp::PatchRange pr{mi, b0, b1, (uint32_t)halo.size(), 0u, (uint32_t)pfaces.size(), 0u, 0u};
-        if (ms_.patch_ok) {
-          const size_t h0 = halo.size(), q0 = pfaces.size();
-          auto local = [&](uint32_t g) -> uint32_t {  // LDS index of device vertex g (adds it to the halo)
-            if (g >= b0 && g < b1) return g - b0;
-            if (mark[g] < 0) { mark[g] = (int32_t)(halo.size() - h0); halo.push_back(g); }
-            return (b1 - b0) + (uint32_t)mark[g];
-          };
-          for (uint32_t i = b0; i < b1; ++i)
-            for (uint32_t j = 0; j < scnt[i]; ++j) {
-              const uint32_t e = sfirst[i] + fp::kSlice * j;
-              sloc[e] = (uint16_t)std::min<uint32_t>(local(sent[e].nbr), fp::kPatchLocal - 1);
-            }
-          std::vector<uint32_t> fl;
-          for (uint32_t i = b0; i < b1; ++i)
-            for (uint32_t k = vf_row[i - ms_.vbase]; k < vf_row[i - ms_.vbase + 1]; ++k)
-              if (!fseen[vf[k]]) { fseen[vf[k]] = 1; fl.push_back(vf[k]); }
-          for (uint32_t qi = 0; qi < (uint32_t)fl.size(); ++qi) {
-            const uint32_t f = fl[qi];
-            fp::PatchFace pf{};
-            uint32_t g3[3];
-            for (int c = 0; c < 3; ++c) {
-              g3[c] = (uint32_t)ms_.dev(h.f_v[3 * (uint64_t)f + c]);
-              pf.v[c] = (uint16_t)std::min<uint32_t>(local(g3[c]), fp::kPatchLocal - 1);
-            }
-            // corner c of a patch vertex: Face::vertex_angle's operands are the next and the
-            // previous corner (springy_mesh.rs:163-172), in the entry of the face's rank among
-            // that vertex's faces
-            for (int c = 0; c < 3; ++c)
-              if (g3[c] >= b0 && g3[c] < b1)
-                pcorner[ffirst[g3[c]] + fp::kSlice * frank[3 * (ms_.fbase + f) + c]] =
-                    fp::PatchCorner{(uint16_t)std::min<uint32_t>(qi, fp::kPatchFaceCap - 1), pf.v[(c + 1) % 3],
-                                    pf.v[(c + 2) % 3], 0};
-            pfaces.push_back(pf);
-          }
-          for (uint32_t f : fl) fseen[f] = 0;
-          pr.n_halo = (uint32_t)(halo.size() - h0);
-          pr.n_faces = (uint32_t)(pfaces.size() - q0);
-          if ((b1 - b0) + pr.n_halo > fp::kPatchLocal || pr.n_faces > fp::kPatchFaceCap) ms_.patch_ok = false;
-          for (size_t k = h0; k < halo.size(); ++k) mark[halo[k]] = -1;
-        }
-        s->h_patches.push_back(pr);
-      }
-    }
-    pfaces.push_back(fp::PatchFace{});
-    HIP_TRY(s->patch_faces.upload(pfaces.data(), pfaces.size()));
-    HIP_TRY(s->p_corner.upload(pcorner.data(), pcorner.size()));
-    {
-      std::vector<fp::PatchStrut> pent(n_sent + 1);
-      for (uint64_t e = 0; e <= n_sent; ++e) pent[e] = fp::PatchStrut{sent[e].len, sloc[e], scls[e]};
-      HIP_TRY(s->p_ent.upload(pent.data(), pent.size()));
-      HIP_TRY(s->p_cls.upload(cls_tab.data(), cls_tab.size()));
-    }
-    halo.push_back(0);
-    HIP_TRY(s->halo.upload(halo.data(), halo.size()));
-    HIP_TRY(s->s_first.upload(sfirst.data(), sfirst.size()));
-    HIP_TRY(s->s_cnt.upload(scnt.data(), scnt.size()));
-    HIP_TRY(s->s_ent.upload(sent.data(), sent.size()));
-    HIP_TRY(s->face_c.alloc(n_fent + 1));
-    HIP_TRY(s->f_first.upload(ffirst.data(), ffirst.size()));
-    HIP_TRY(s->f_cnt.upload(fcnt.data(), fcnt.size()));
-    HIP_TRY(s->f_slot.upload(fslot.data(), fslot.size()));
-    HIP_TRY(s->f_v.upload(fv.data(), fv.size()));
-  }
-
-  // torsional hinges (springy_mesh.rs:348-393 builds them; :555-567 names the four vertices)
-  {
-    std::vector<fp::HingeEnt> hs;
-    for (auto& ms_ : s->meshes) {
-      const auto& h = ms_.host;
-      ms_.hinge_begin = (uint32_t)hs.size();
-      for (uint64_t k = 0; k < h.n_struts(); ++k) {
-        if (!h.t_has[k] || h.s_f0[k] < 0 || h.s_f1[k] < 0) continue;
-        const uint32_t a = h.s_v0[k], b = h.s_v1[k];
-        auto other = [&](int64_t f) -> uint32_t {  // utils::tuple_difference(face vertices, strut)
-          for (int j = 0; j < 3; ++j) {
-            const uint32_t v = (uint32_t)h.f_v[3 * (size_t)f + j];
-            if (v != a && v != b) return v;
-          }
-          return a;
-        };
-        fp::HingeEnt e{};
-        e.v[0] = (uint32_t)ms_.dev(a);
-        e.v[1] = (uint32_t)ms_.dev(b);
-        e.v[2] = (uint32_t)ms_.dev(other(h.s_f0[k]));
-        e.v[3] = (uint32_t)ms_.dev(other(h.s_f1[k]));
-        e.k = h.t_k[k];
-        e.c = h.t_c[k];
-        e.rest = h.t_rest[k];
-        hs.push_back(e);
-      }
-      ms_.hinge_end = (uint32_t)hs.size();
-    }
-    s->n_hinges = (uint32_t)hs.size();
-    if (!hs.empty()) {
-      std::vector<uint32_t> trow(N + 1, 0);
-      for (const auto& e : hs)
-        for (int r = 0; r < 4; ++r) trow[e.v[r] + 1]++;
-      for (uint64_t i = 0; i < N; ++i) trow[i + 1] += trow[i];
-      std::vector<uint32_t> tent(trow[N]), tfill(trow.begin(), trow.end() - 1);
-      for (uint32_t k = 0; k < hs.size(); ++k)
-        for (uint32_t r = 0; r < 4; ++r) tent[tfill[hs[k].v[r]]++] = (k << 2) | r;
-      HIP_TRY(s->hinges.upload(hs.data(), hs.size()));
-      HIP_TRY(s->t_row.upload(trow.data(), trow.size()));
-      HIP_TRY(s->t_ent.upload(tent.data(), tent.size()));
-      HIP_TRY(s->hinge_f.alloc(4 * hs.size()));
-    }
-  }
-
-  // cloth descriptors, block table
-  {
-    std::vector<fp::ClothView> cv;
-    for (uint32_t mi = 0; mi < s->meshes.size(); ++mi) {
-      MeshSlot& ms_ = s->meshes[mi];
-      const fp::ClothDesc& d = ms_.host.cloth;
-      fp::ClothView c{};
-      c.base = (uint32_t)ms_.vbase;
-      c.rows = d.rows;
-      c.cols = d.cols;
-      c.spacing = d.spacing;
-      c.cx = (float)d.cols / 2.0f * d.spacing;
-      c.cy = (float)d.rows / 2.0f * d.spacing;
-      c.p0 = d.position[0];
-      c.p1 = d.plane == FP_PLANE_XZ ? d.position[2] : d.position[1];
-      c.tk = d.tk; c.tc = d.tc; c.sk = d.sk; c.sc = d.sc; c.bk = d.bk; c.bc = d.bc;
-      cv.push_back(c);
-      ms_.blk_begin = (uint32_t)s->h_blocks.size();
-      for (size_t b = 0; b + 1 < ms_.bstart.size(); ++b)
-        s->h_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.vbase + ms_.bstart[b]),
-                                             (uint32_t)(ms_.vbase + ms_.bstart[b + 1]), 0u});
-      ms_.blk_end = (uint32_t)s->h_blocks.size();
-      ms_.fblk_begin = (uint32_t)s->h_face_blocks.size();
-      const uint64_t nf = ms_.host.n_faces();
-      for (uint64_t b = 0; b < nf; b += fp::kBlock)
-        s->h_face_blocks.push_back(fp::BlockRange{mi, (uint32_t)(ms_.fbase + b),
-                                                  (uint32_t)(ms_.fbase + std::min<uint64_t>(nf, b + fp::kBlock)), 0u});
-      ms_.fblk_end = (uint32_t)s->h_face_blocks.size();
-    }
-    HIP_TRY(s->cloths.upload(cv.data(), cv.size()));
-    HIP_TRY(s->all_blocks.upload(s->h_blocks.data(), s->h_blocks.size()));
-    HIP_TRY(s->all_face_blocks.upload(s->h_face_blocks.data(), s->h_face_blocks.size()));
-  }
+  if (int e = build_incidence(s.get())) return e;
+  if (int e = build_hinges(s.get())) return e;
+  if (int e = build_cloth_views(s.get())) return e;
   std::vector<fp::HostObstacle> obs;
   for (auto& o : os) obs.push_back(std::move(o->o));
   if (int e = fph::upload_obstacles(s->obst, obs)) return e;
