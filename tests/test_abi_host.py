@@ -3,6 +3,7 @@ constructors (no GPU needed) build the same tables as the oracle, bit for bit.""
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -64,6 +65,14 @@ def test_cpp_mirror_reads_like_the_reference_tests(tmp_path):
                     "-Wl,-rpath," + libdir, "-o", exe], check=True)
     out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip()
     assert out == "ok", out
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: asked for more it says so and exits with status 2, before it
+    touches a device."""
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU only" in r.stderr and r.stdout == ""
 
 
 def test_rust_binding_declares_every_entry_point():
