@@ -26,9 +26,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VERTEX_STEP = 48
 # HBM bytes per launch of the default step kernel (grid_tiled_kernel<ExactMath, 16, 16>) on the
-# 1024 x 1024 cloth, from rocprofv3 FETCH_SIZE (13,212 KiB x 2.000) and WRITE_SIZE (24,576 KiB x
+# 1024 x 1024 cloth, from rocprofv3 FETCH_SIZE (13,249 KiB x 2.000) and WRITE_SIZE (24,576 KiB x
 # 1.000), each scaled by a 256 MiB device copy's reading (profiles/grid_tiled_exact_cloth1024.md)
-MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13212 * 2.0 + 24576 * 1.0) * 1024}
+MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13249 * 2.0 + 24576 * 1.0) * 1024}
 # VALU issue of the same kernel from rocprofv3 SQ_INSTS_VALU (per wave) and GRBM_GUI_ACTIVE (per XCD):
 # 1,744 instructions x 4 cycles x 16 waves per SIMD against 124.9 k active cycles
 # (profiles/grid_tiled_cloth1024_pmc.md); the step is bound by VALU issue, not by HBM
