@@ -30,10 +30,10 @@ BYTES_PER_VERTEX_STEP = 48
 # 1.000), each scaled by a 256 MiB device copy's reading (profiles/grid_tiled_exact_cloth1024.md)
 MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13249 * 2.0 + 24576 * 1.0) * 1024}
 # VALU issue of the same kernel from rocprofv3 SQ_INSTS_VALU (per wave) and GRBM_GUI_ACTIVE (per XCD):
-# 1,744 instructions x 4 cycles x 16 waves per SIMD against 124.9 k active cycles
+# 2,177 instructions x 4 cycles x 16 waves per SIMD against 140.8 k active cycles
 # (profiles/grid_tiled_cloth1024_pmc.md); the step is bound by VALU issue, not by HBM
-MEASURED_VALU = {(1024, "exact", "auto", "default"): {"insts_per_wave": 1744, "issue_cycles_per_simd": 111616,
-                                                      "active_cycles": 124875, "issue_frac": 111616 / 124875}}
+MEASURED_VALU = {(1024, "exact", "auto", "default"): {"insts_per_wave": 2177, "issue_cycles_per_simd": 139328,
+                                                      "active_cycles": 140839, "issue_frac": 139328 / 140839}}
 # the same for patch_kernel<ExactMath> on the 64-icosphere workload (FETCH_SIZE 216,682 KiB x 2.000,
 # WRITE_SIZE 50,318 KiB x 1.000; profiles/batched_and_surfaces.md)
 MEASURED_TRAFFIC_SURFACES = {"exact": (216682 * 2.0 + 50318 * 1.0) * 1024}
