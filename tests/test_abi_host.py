@@ -246,3 +246,56 @@ def test_surrogate_config_default():
     assert c.base_struts == 0 and c.strain_limit == np.float32(0.1) and c.energy_limit == np.float32(0.01)
     assert fa.SurrogateConfig().to_c().out_scale == pytest.approx(c.out_scale, rel=1e-6)
 
+
+
+_ASAN_SCRIPT = r"""
+import sys
+import numpy as np
+import feriphys_amd._lib as L
+L.LIB_PATH = sys.argv[1]
+import feriphys_amd as fa
+from feriphys_amd import workloads as wl
+cl = fa.Cloth(48, 40, 0.05, (0.0, 0.0, 0.0), 0.01, 500.0, 2.0, 300.0, 1.0, 50.0, 0.5, [0, 39])
+m = cl.mesh
+v01, kcl, faces = m.struts()
+assert m.counts()[1] == 48 * 39 + 47 * 40 + 47 * 39 + 2 * 46 * 38  # tensile, one shear per quad, bends
+rec = m.get_points()
+m.set_points(rec)
+pos, idx = wl.icosphere(3)
+s = fa.SpringyMesh(pos, idx, 1.0, 10.0, 0.5)
+for a, b in [(0, 5), (3, 9)]:
+    s.add_strut((a, b), 2.0, 0.1)
+s.add_pin(2)
+s.struts(); s.faces(); s.pins(); s.get_vertices()
+o = fa.CollidableMesh(*wl.icosphere(2))
+o.counts()
+print("ok")
+"""
+
+
+def test_host_code_under_address_sanitizer(tmp_path):
+    """The library's host code (constructors, tables, readback of host meshes, handles and their
+    release) built with AddressSanitizer (make asan; device code unchanged) and driven by the C and
+    C++ caller programs, with leak detection, and by the Python mirror with the sanitizer runtime
+    preloaded: no invalid access, double free or leak is reported."""
+    csrc = os.path.join(ROOT, "feriphys_amd", "csrc")
+    subprocess.run(["make", "-s", "-j8", "-C", csrc, "asan"], check=True, capture_output=True)
+    lib = os.path.join(ROOT, "_build", "asan", "libferiphys_gpu.so")
+    libdir = os.path.dirname(lib)
+    llvm = "/opt/rocm/llvm/bin"
+    rt = subprocess.run([os.path.join(llvm, "clang"), "-print-file-name=libclang_rt.asan-x86_64.so"],
+                        capture_output=True, text=True, check=True).stdout.strip()
+    assert os.path.exists(rt), rt
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1")
+    for src, cc, std in (("abi_c_check.c", "clang", "-std=c99"), ("mirror_check.cpp", "clang++", "-std=c++17")):
+        exe = str(tmp_path / src.split(".")[0])
+        subprocess.run([os.path.join(llvm, cc), std, "-g", "-fsanitize=address", "-I", os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "native", src), "-L", libdir, "-lferiphys_gpu",
+                        "-Wl,-rpath," + libdir, "-o", exe], check=True)
+        r = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=300)
+        assert r.returncode == 0 and r.stdout.strip() == "ok", (src, r.stdout[-2000:], r.stderr[-4000:])
+    # the interpreter itself is not instrumented: leaks off, the runtime preloaded
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1:verify_asan_link_order=0", LD_PRELOAD=rt,
+               PYTHONPATH=ROOT + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    r = subprocess.run([sys.executable, "-c", _ASAN_SCRIPT, lib], capture_output=True, text=True, env=env, timeout=600)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout[-2000:], r.stderr[-4000:])
