@@ -94,7 +94,9 @@ int fp_mesh_new(const float* positions, uint64_t n_vertices, const uint64_t* ind
                 const fp_strut_override* overrides, uint64_t n_overrides, fp_mesh** out);
 
 /* Cloth::new (springy/cloth.rs:20-113): rows x cols grid in the xy plane, centred on `position`;
- * tensile/shear/bending ("binding") nominal spring constants; pinned vertex indices. */
+ * tensile/shear/bending ("binding") nominal spring constants; pinned vertex indices.  rows or cols
+ * below 2 returns FP_EINVAL: the reference's bend loops run over 0..(rows - 2) and 0..(cols - 2)
+ * (cloth.rs:101-102), which underflow and panic there. */
 int fp_cloth_new(uint64_t rows, uint64_t cols, float spacing, const float position[3], float point_mass,
                  float tensile_stiffness, float tensile_damping, float shear_stiffness, float shear_damping,
                  float binding_stiffness, float binding_damping, const uint64_t* pinned_vertices,

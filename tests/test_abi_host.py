@@ -145,6 +145,20 @@ def test_cloth_counts_closed_form(rows, cols):
     assert f == 2 * (rows - 1) * (cols - 1)
 
 
+@pytest.mark.parametrize("rows,cols", [(1, 5), (5, 1), (1, 1), (2, 1), (0, 4)])
+def test_cloth_below_two_by_two_is_refused(rows, cols):
+    """Cloth::new's bend loops run over 0..(rows - 2) and 0..(cols - 2) (springy/cloth.rs:101-102),
+    which underflow and panic for a side below 2: the library returns FP_EINVAL and the oracle
+    refuses the same sizes."""
+    c = wl.config0(rows=2, cols=2).cloths[0]
+    args = (rows, cols) + tuple(c.args())[2:-1] + ([],)
+    with pytest.raises(fa.FpError) as e:
+        fa.Cloth(*args)
+    assert e.value.code == _lib.FP_EINVAL
+    with pytest.raises(ValueError):
+        orc.Mesh.cloth(*args)
+
+
 def test_three_by_three_cloth_strut_list_by_hand():
     """Brute force on a tiny input: every strut of the 3x3 cloth, enumerated by hand from the
     triangle list (top-left triangles (i+1, i+3, i) of quads 0, 1, 3, 4, then bottom-right
