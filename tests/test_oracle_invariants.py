@@ -238,3 +238,155 @@ def test_f32_oracle_tracks_f64_over_600_steps():
     b.step(600)
     e = np.abs(a.mesh(0).points()[:, 1:4].astype(f64) - b.mesh(0).points()[:, 1:4]).max()
     assert e < 1e-3
+
+
+# ---- torsional hinges (springy_mesh.rs:553-627): pinned by the energy they derive from ----------
+#
+# A hinge spring of stiffness k and damping c about the strut x0 -> x1 stores E = k (theta - rest)^2 / 2
+# and dissipates with the angle's rate theta' (the reference takes theta' from the velocities of the
+# two wing vertices x2, x3 only).  Physically its forces are F_i = -(k (theta - rest) + c theta') grad_i
+# theta.  The dihedral angle theta(x0, x1, x2, x3) is the quantity the reference measures (the signed
+# angle between the wing normals about the hinge direction); its gradient here comes from central
+# finite differences in double precision.  The reference's wing forces F2, F3 are exactly that.  Its
+# edge force is not: springy_mesh.rs:608 computes force_1 = (d_02 force_2 + d_03 force_3) / l_01,
+# the opposite sign of the moment balance about x0 (which needs F1 = -(d02 F2 + d03 F3) / L), so its
+# four forces sum to zero but their moments do not (the reference disables the hinges in its step,
+# :518-519, noting that they make meshes explode).  The oracle restates the reference, quirk
+# included: F1 is pinned to MINUS the physical gradient term and F0 to -(F1 + F2 + F3).  A flipped
+# sign, a dropped 1/|r| or a wrong lever arm in the restated forces fails these comparisons.
+
+def _hinge_angle(x0, x1, x2, x3):
+    h = (x1 - x0) / np.linalg.norm(x1 - x0)
+    nl = np.cross(x1 - x0, x2 - x0)
+    nl /= np.linalg.norm(nl)
+    nr = np.cross(x3 - x0, x1 - x0)
+    nr /= np.linalg.norm(nr)
+    return math.atan2(float(np.dot(np.cross(nl, nr), h)), float(np.dot(nl, nr)))
+
+
+def _hinge_mesh(x, dtype, k, c):
+    """Two triangles sharing the edge (0, 1): vertex 2 on the first face, vertex 3 on the second."""
+    m = orc.Mesh.new(np.asarray(x, dtype), [0, 1, 2, 1, 0, 3], 1.0, 0.0, 0.0, torsion=(k, c), dtype=dtype)
+    st = m.struts()
+    hinge = [s for s in range(len(st["v01"])) if st["has_torsion"][s]]
+    assert len(hinge) == 1
+    s = hinge[0]
+    assert sorted(st["v01"][s].tolist()) == [0, 1]
+    return m, s, st
+
+
+def _hinge_roles(st, s, faces_v3):
+    """(x0, x1, x2, x3) vertex ids as the reference assigns them: the strut's (v0, v1), then the
+    vertex of its first face and of its second face that is not on the strut."""
+    v0, v1 = (int(v) for v in st["v01"][s])
+    f0, f1 = (int(f) for f in st["faces"][s])
+    w2 = [int(v) for v in faces_v3[f0] if v not in (v0, v1)][0]
+    w3 = [int(v) for v in faces_v3[f1] if v not in (v0, v1)][0]
+    return v0, v1, w2, w3
+
+
+def test_hinge_closed_form_axis_aligned():
+    """Hinge along +z from the origin, wing x2 at lever arm a on +x, wing x3 at lever arm b at angle
+    phi about z.  With zero velocities: |F2| = k delta / a along the left normal (0, 1, 0),
+    |F3| = k delta / b along the right normal (sin phi, -cos phi, 0), F1 = +(d02 F2 + d03 F3) / L (the
+    reference's sign, springy_mesh.rs:608; the moment balance would give the opposite), F0 =
+    -(F1 + F2 + F3); delta = theta - rest, theta = phi - pi."""
+    L, a, b, d2, d3, phi, k = 2.0, 0.5, 0.8, 0.6, 1.3, 2.0, 3.0
+    x = np.array([[0, 0, 0], [0, 0, L], [a, 0, d2], [b * math.cos(phi), b * math.sin(phi), d3]], f64)
+    m, s, st = _hinge_mesh(x, f64, k, 0.0)
+    v0, v1, w2, w3 = _hinge_roles(st, s, m.faces()[1])
+    assert (v0, v1) == (0, 1) and (w2, w3) == (2, 3)
+    rest = 0.25
+    m.set_torsion_rest(s, rest)
+    m.clear_forces()
+    m.apply_torsional_forces()
+    F = m.points()[:, 7:10]
+    theta = phi - math.pi
+    assert abs(_hinge_angle(*x) - theta) < 1e-12
+    delta = theta - rest
+    F2 = k * delta / a * np.array([0.0, 1.0, 0.0])
+    F3 = k * delta / b * np.array([math.sin(phi), -math.cos(phi), 0.0])
+    F1 = (d2 * F2 + d3 * F3) / L
+    F0 = -(F1 + F2 + F3)
+    for got, want in ((F[0], F0), (F[1], F1), (F[2], F2), (F[3], F3)):
+        assert np.abs(got - want).max() < 1e-12 * k, (got, want)
+
+
+def test_hinge_restoring_sign():
+    """Moving each wing a little along the force the hinge puts on it brings the angle back towards
+    its rest angle, from either side of it."""
+    L, a, b, phi, k = 1.0, 0.4, 0.7, 2.4, 5.0
+    x = np.array([[0, 0, 0], [0, 0, L], [a, 0, 0.3], [b * math.cos(phi), b * math.sin(phi), 0.2]], f64)
+    m, s, st = _hinge_mesh(x, f64, k, 0.0)
+    theta = _hinge_angle(*x)
+    for rest in (theta - 0.3, theta + 0.3):
+        m.set_torsion_rest(s, rest)
+        m.clear_forces()
+        m.apply_torsional_forces()
+        F = m.points()[:, 7:10]
+        for i in (2, 3):
+            y = x.copy()
+            y[i] += 1e-4 * F[i] / np.linalg.norm(F[i])
+            assert abs(_hinge_angle(*y) - rest) < abs(theta - rest), (rest, i)
+
+
+@pytest.mark.parametrize("dtype,tol", [(f64, 1e-6), (f32, 2e-3)])
+def test_hinge_forces_are_the_energy_gradient(dtype, tol):
+    """Random bent hinges, wing velocities on: F_i = -(k delta + c theta') grad_i theta for all four
+    vertices, theta' = grad_2 theta . v2 + grad_3 theta . v3; gradients by central differences."""
+    rng = np.random.default_rng(7)
+    for trial in range(12):
+        x0 = rng.normal(size=3)
+        e = rng.normal(size=3)
+        e *= rng.uniform(0.5, 2.0) / np.linalg.norm(e)
+        x1 = x0 + e
+        # wings: a point along the edge plus an offset perpendicular to it
+        def wing():
+            w = rng.normal(size=3)
+            w -= np.dot(w, e) / np.dot(e, e) * e
+            w *= rng.uniform(0.3, 1.5) / np.linalg.norm(w)
+            return x0 + rng.uniform(-0.3, 1.3) * e + w
+        x = np.array([x0, x1, wing(), wing()], f64)
+        k, c = rng.uniform(0.5, 4.0), rng.uniform(0.0, 2.0)
+        m, s, st = _hinge_mesh(x, dtype, k, c)
+        v0, v1, w2, w3 = _hinge_roles(st, s, m.faces()[1])
+        order = [v0, v1, w2, w3]
+        xr = x[order]
+        theta = _hinge_angle(*xr)
+        if abs(abs(theta) - math.pi) < 0.2:
+            continue  # near the atan2 branch cut the finite differences straddle it
+        rest = theta + rng.uniform(-0.5, 0.5)
+        m.set_torsion_rest(s, rest)
+        vel = np.zeros((4, 3))
+        vel[w2] = rng.normal(size=3)
+        vel[w3] = rng.normal(size=3)
+        rec = m.points().astype(f64)
+        rec[:, 4:7] = vel
+        rec[:, 7:10] = 0.0
+        m.set_points(rec.astype(dtype))
+        m.apply_torsional_forces()
+        F = m.points()[:, 7:10].astype(f64)
+        eps = 1e-6
+        grad = np.zeros((4, 3))
+        for i in range(4):
+            for a in range(3):
+                xp, xm = xr.copy(), xr.copy()
+                xp[i, a] += eps
+                xm[i, a] -= eps
+                grad[i, a] = (_hinge_angle(*xp) - _hinge_angle(*xm)) / (2 * eps)
+        theta_dot = float(np.dot(grad[2], vel[w2]) + np.dot(grad[3], vel[w3]))
+        want = -(k * (theta - rest) + c * theta_dot) * grad
+        scale = np.abs(want).max()
+        got = F[order]
+        # wings: the physical forces
+        assert np.abs(got[2:] - want[2:]).max() / scale < tol, trial
+        # edge end x1: the reference's sign is the opposite of the moment balance
+        assert np.abs(got[1] + want[1]).max() / scale < tol, trial
+        # x0 closes the sum
+        assert np.abs(got[0] + got[1] + got[2] + got[3]).max() / scale < tol, trial
+        if dtype is f64:
+            # the reference's forces carry a net moment about x0 (2 F1 x (x1 - x0) of the physical one)
+            mom = sum(np.cross(xr[i] - xr[0], got[i]) for i in range(4))
+            phys = sum(np.cross(xr[i] - xr[0], want[i]) for i in range(4))
+            assert np.abs(phys).max() < 1e-6 * scale
+            assert np.abs(mom - 2 * np.cross(xr[1] - xr[0], got[1])).max() < 1e-6 * scale
