@@ -3,17 +3,27 @@ sphere collider on one MI355X.
 
 One "step" is one Simulation::step of the whole cloth (forces, RK4, collision, pins) at
 dt = 1/600 s.  The cloth's state is resident in HBM before the timed region; the timed region is
-exactly K steps bracketed by device synchronisations.
+exactly K steps bracketed by device synchronisations, replayed from captured HIP graphs.
+
+Both math forms of the step are timed: EXACT (the reference's f32 results bit for bit) and FAST
+(one-step forces and states within 1e-5 relative of the reference; the parity tests hold it to
+that).  The headline is the faster form that meets its bar, or the learned surrogate when its
+600-step run stays within 1e-3 of the exact plain run and is faster still.
 
 Prints ONE JSON line:
-  value           simulated seconds per wall second (K * dt / wall time)
-  roofline        the step kernel against HBM bandwidth: 48 algorithmic bytes per vertex-step
-                  (x, y, z, vx, vy, vz read once and written once, f32) over the kernel's average
-                  duration measured with HIP events on the simulation's stream
+  value           simulated seconds per wall second (K * dt / wall time) of the headline form
+  roofline        the headline's step kernel against HBM bandwidth: 48 algorithmic bytes per
+                  vertex-step (x, y, z, vx, vy, vz read once and written once, f32) over the
+                  kernel's average duration measured with HIP events on the simulation's stream
+  surrogate       the learned big step over the full 600-step horizon (75 big steps of 8 dt),
+                  compared with the exact plain run after the same number of dt
+  batched,        the plain rates of the 4,096-cloth batch and the 64-icosphere workload, with
+  surfaces        their rooflines (omitted with --no-extra)
   cpu_baseline    the oracle (a single-threaded C++ restatement of the reference step) on a bounded
                   sample of the same workload, on this host's cores
 """
 import argparse
+import hashlib
 import json
 import os
 import platform
@@ -25,18 +35,106 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_VERTEX_STEP = 48
-# HBM bytes per launch of the default step kernel (grid_tiled_kernel<ExactMath, 16, 16>) on the
-# 1024 x 1024 cloth, from rocprofv3 FETCH_SIZE (13,249 KiB x 2.000) and WRITE_SIZE (24,576 KiB x
-# 1.000), each scaled by a 256 MiB device copy's reading (profiles/grid_tiled_exact_cloth1024.md)
-MEASURED_TRAFFIC = {(1024, "exact", "auto", "default"): (13249 * 2.0 + 24576 * 1.0) * 1024}
-# VALU issue of the same kernel from rocprofv3 SQ_INSTS_VALU (per wave) and GRBM_GUI_ACTIVE (per XCD):
-# 2,177 instructions x 4 cycles x 16 waves per SIMD against 140.8 k active cycles
-# (profiles/grid_tiled_cloth1024_pmc.md); the step is bound by VALU issue, not by HBM
-MEASURED_VALU = {(1024, "exact", "auto", "default"): {"insts_per_wave": 2177, "issue_cycles_per_simd": 139328,
-                                                      "active_cycles": 140839, "issue_frac": 139328 / 140839}}
-# the same for patch_kernel<ExactMath> on the 64-icosphere workload (FETCH_SIZE 216,682 KiB x 2.000,
-# WRITE_SIZE 50,318 KiB x 1.000; profiles/batched_and_surfaces.md)
-MEASURED_TRAFFIC_SURFACES = {"exact": (216682 * 2.0 + 50318 * 1.0) * 1024}
+SURROGATE_HORIZON = 600  # plain steps the surrogate leg covers, whatever --steps is
+CSRC = os.path.join(ROOT, "feriphys_amd", "csrc")
+COMMON = ["step_device.hpp", "vec.hpp", "exact_ops.hpp", "fdlibm_atan2f.hpp", "kernels.hpp"]
+# the source files each measured kernel is built from; a counter reading below is reported only
+# while these files still hash to the value it was measured on
+KERNEL_SOURCES = {
+    "grid_tiled_kernel<ExactMath,16,16>": ["grid_tiled.hip"] + COMMON,
+    "patch_kernel<ExactMath>": ["kernels.hip"] + COMMON,
+}
+# rocprofv3 readings per launch: FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the same
+# counter reads for a 256 MiB device copy (2.000 and 1.000), and the VALU issue of the kernel
+# (SQ_INSTS_VALU per wave, GRBM_GUI_ACTIVE per XCD)
+MEASURED = {
+    ("grid_tiled_kernel<ExactMath,16,16>", "cloth1024"): {
+        "src_sha": "8e8d8a1c89687b1b", "fetch_kib": 13249, "write_kib": 24576,
+        "valu_insts_per_wave": 2177, "waves_per_simd": 16, "active_cycles": 140839,
+        "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
+    ("patch_kernel<ExactMath>", "icospheres64"): {
+        "src_sha": "581cd32381980017", "fetch_kib": 216682, "write_kib": 50318,
+        "profiles": ["profiles/batched_and_surfaces.md"]},
+}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
+def src_sha(kernel):
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES[kernel]:
+        with open(os.path.join(CSRC, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def measured(kernel, workload):
+    """The counter readings for (kernel, workload) if they were taken on the current sources."""
+    m = MEASURED.get((kernel, workload))
+    if m is None or kernel not in KERNEL_SOURCES or src_sha(kernel) != m["src_sha"]:
+        return None
+    return m
+
+
+def traffic_of(kernel, workload):
+    m = measured(kernel, workload)
+    return None if m is None else (m["fetch_kib"] * FETCH_SCALE + m["write_kib"] * WRITE_SCALE) * 1024
+
+
+def valu_of(kernel, workload):
+    m = measured(kernel, workload)
+    if m is None or "valu_insts_per_wave" not in m:
+        return None
+    issue = m["valu_insts_per_wave"] * 4 * m["waves_per_simd"]
+    return {"insts_per_wave": m["valu_insts_per_wave"], "issue_cycles_per_simd": issue,
+            "active_cycles": m["active_cycles"], "issue_frac": issue / m["active_cycles"], "profiles": m["profiles"]}
+
+
+def graph_chunk(steps, chunk):
+    """Steps per captured graph: at most `chunk` and at most `steps`, even (so one graph serves
+    both state-buffer parities' starting point), at least 2."""
+    c = max(2, min(chunk, steps))
+    return c - (c % 2)
+
+
+def time_plain(sim, initial_per_mesh, steps, warmup, chunk):
+    """Times `steps` steps replayed from `chunk`-step HIP graphs, starting from the given records:
+    an untimed warm-up of at least one chunk (which captures the graph), a restore of the initial
+    records (which keeps the captured graph), then the timed steps, bracketed by synchronisations;
+    HIP events on the simulation's stream measure the device time of the same steps."""
+    def restore():
+        for k, rec in enumerate(initial_per_mesh):
+            sim.set_points(k, rec)
+    restore()
+    w = max(warmup, chunk)
+    sim.step_graph(-(-w // chunk) * chunk, chunk)
+    restore()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms = sim.time_steps_graph(steps, chunk)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, kernel_ms
+
+
+def plain_leg(sim, fa, kern, math, initial, steps, warmup, chunk, N, dt, np):
+    sim.set_kernel(kern, fa.FP_MATH_EXACT if math == "exact" else fa.FP_MATH_FAST)
+    wall, kernel_ms = time_plain(sim, [initial], steps, warmup, chunk)
+    pts = sim.get_points(0)
+    kavg = kernel_ms * 1e-3 / steps
+    return {"math": math, "sim_s_per_s": steps * dt / wall, "ms_per_step": wall * 1e3 / steps,
+            "vertex_steps_per_s": steps * N / wall, "kernel_avg_us": kavg * 1e6,
+            "achieved_gbs": BYTES_PER_VERTEX_STEP * N / kavg / 1e9,
+            "hbm_frac": BYTES_PER_VERTEX_STEP * N / kavg / 1e9 / HBM_PEAK_GBS,
+            "finite": bool(np.isfinite(pts).all())}, pts
+
+
+def run_from(sim, fa, kern, math, initial, steps, chunk):
+    """Positions (f64) after `steps` plain steps from `initial` under the given math form."""
+    import numpy as np
+    sim.set_kernel(kern, fa.FP_MATH_EXACT if math == "exact" else fa.FP_MATH_FAST)
+    sim.set_points(0, initial)
+    sim.step_graph(steps, chunk)
+    return sim.get_points(0)[:, 1:4].astype(np.float64)
 
 
 def main():
@@ -45,7 +143,8 @@ def main():
     ap.add_argument("--steps", type=int, default=600)
     ap.add_argument("--warmup", type=int, default=60)
     ap.add_argument("--n", type=int, default=1024, help="cloth side (vertices)")
-    ap.add_argument("--math", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--math", choices=["auto", "exact", "fast"], default="auto",
+                    help="auto: time both forms and headline the faster")
     ap.add_argument("--kernel", choices=["auto", "grid", "grid_simple", "csr"], default="auto")
     ap.add_argument("--tile", choices=["default", "15x15", "7x31", "16x16"], default="default")
     ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
@@ -53,7 +152,11 @@ def main():
                     help="cloth: one 1024x1024 cloth + sphere (default); batched: 4096 cloths of uneven sizes; "
                          "surfaces: 64 perturbed icospheres (2.1M vertices) on a ground plane; rigid: 1M unit "
                          "cubes tumbling onto a ground plane")
-    ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph (batched)")
+    ap.add_argument("--only", choices=["all", "plain", "mlp"], default="all",
+                    help="plain: only the timed plain steps of the cloth (for profiling the step kernel); "
+                         "mlp: only the surrogate network kernel over the cloth's vertices, --steps launches")
+    ap.add_argument("--no-extra", action="store_true", help="leave the batched and surfaces keys out of the line")
+    ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph")
     ap.add_argument("--weights", default=os.path.join(ROOT, "feriphys_amd", "weights", "cloth_surrogate.npy"),
                     help="surrogate network for the cloth workload's surrogate leg ('' to skip it)")
     ap.add_argument("--sur-shape", type=int, choices=[32, 64], default=32,
@@ -75,62 +178,56 @@ def main():
         return bench_surfaces(args, np, fa, wl)
     if args.workload == "rigid":
         return bench_rigid(args, np, fa, wl)
+    if args.only == "mlp":
+        return only_mlp(args, np, fa, wl)
+
     scene = wl.config1(n=args.n, steps=args.steps)
     sim = wl.build_gpu(scene)
     kern = {"auto": fa.FP_KERNEL_AUTO, "grid": fa.FP_KERNEL_GRID, "grid_simple": fa.FP_KERNEL_GRID_SIMPLE,
             "csr": fa.FP_KERNEL_CSR}[args.kernel]
-    sim.set_kernel(kern, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
     if args.tile != "default":
         sim.set_tile_shape(0, {"15x15": 0, "7x31": 1, "16x16": 2}[args.tile])
     N = scene.cloths[0].rows * scene.cloths[0].cols
     dt = scene.config.dt
-
-    # Warm up on the same scene, then restore its initial state so that the timed region is
-    # exactly steps 1..K of the workload (the restore is outside the timed region).
+    chunk = graph_chunk(args.steps, args.chunk)
     initial = sim.get_points(0)
-    if args.warmup:
-        sim.step(args.warmup)
-        sim.set_points(0, initial)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    kernel_ms = sim.time_steps(args.steps)  # HIP events on the simulation's stream around the K steps
-    sim.synchronize()
-    wall = time.perf_counter() - t0
 
-    pts = sim.get_points(0)
-    if not np.isfinite(pts).all():
+    maths = ["exact", "fast"] if args.math == "auto" else [args.math]
+    legs, finals = {}, {}
+    for m in maths:
+        legs[m], finals[m] = plain_leg(sim, fa, kern, m, initial, args.steps, args.warmup, chunk, N, dt, np)
+        print(f"[bench] plain {m}: {legs[m]['kernel_avg_us']:.1f} us/step (HIP events), "
+              f"{legs[m]['sim_s_per_s']:.2f} sim-s/s", file=sys.stderr, flush=True)
+    if not all(l["finite"] for l in legs.values()):
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(3)
+    best = max(legs, key=lambda m: legs[m]["sim_s_per_s"])
+    kname = {"exact": "grid_tiled_kernel<ExactMath,16,16>", "fast": "grid_fast_kernel<15,15>"}
+    for m in legs:
+        legs[m]["kernel"] = kname[m]
+        legs[m]["traffic"] = traffic_of(kname[m], "cloth1024") if args.n == 1024 else None
+        legs[m]["valu_issue"] = valu_of(kname[m], "cloth1024") if args.n == 1024 else None
 
-    sim_s_per_s = args.steps * dt / wall
-    ms_per_step = wall * 1e3 / args.steps
-    kernel_avg_s = kernel_ms * 1e-3 / args.steps
-    achieved = BYTES_PER_VERTEX_STEP * N / kernel_avg_s / 1e9
+    if args.only == "plain":
+        print(json.dumps(headline_line(args, legs, best, None, None, {}, N, dt, scene)), flush=True)
+        return
 
-    # the same 600 steps with the fast math policy (one-step forces within 1e-5 of the exact ones;
-    # 600-step trajectories checked against the f64 oracle in the parity tests)
-    fast = None
-    if args.math == "exact":
-        sim.set_kernel(kern, fa.FP_MATH_FAST)
-        sim.set_points(0, initial)
-        sim.step(max(1, args.warmup))
-        sim.set_points(0, initial)
-        sim.synchronize()
-        tf = time.perf_counter()
-        fast_ms = sim.time_steps(args.steps)
-        sim.synchronize()
-        fwall = time.perf_counter() - tf
-        xf = sim.get_points(0)[:, 1:4].astype(np.float64)
-        favg = fast_ms * 1e-3 / args.steps
-        fast = {"sim_s_per_s": args.steps * dt / fwall, "kernel_avg_us": favg * 1e6,
-                "hbm_frac": BYTES_PER_VERTEX_STEP * N / favg / 1e9 / HBM_PEAK_GBS,
-                "max_position_difference_vs_exact_m": float(np.abs(xf - pts[:, 1:4]).max())}
-        sim.set_kernel(kern, fa.FP_MATH_EXACT)
+    # the two forms over the surrogate's 600-step horizon: the exact run is the surrogate's judge
+    horizon_chunk = graph_chunk(SURROGATE_HORIZON, args.chunk)
+    ref_exact = run_from(sim, fa, kern, "exact", initial, SURROGATE_HORIZON, horizon_chunk)
+    compare = {"steps": SURROGATE_HORIZON}
+    if "fast" in legs:
+        xf = run_from(sim, fa, kern, "fast", initial, SURROGATE_HORIZON, horizon_chunk)
+        d = np.linalg.norm(xf - ref_exact, axis=1)
+        compare.update({"fast_vs_exact_max_m": float(d.max()),
+                        "fast_vs_exact_rms_over_extent": float(np.sqrt(np.mean(d * d))) / ((args.n - 1) * 0.05)})
 
-    sur = surrogate_leg(args, np, fa, wl, kern, pts, initial, N, dt, sim_s_per_s)
+    sur = surrogate_leg(args, np, fa, wl, sim, kern, ref_exact, initial, N, dt, legs)
 
     # PCIe-inclusive rate (never the headline): a caller that reads the 10-float records back after
     # every step, as the reference's demo renders every frame; host clock over 20 frames
+    sim.set_surrogate(None)
+    sim.set_kernel(kern, fa.FP_MATH_EXACT if best == "exact" else fa.FP_MATH_FAST)
     sim.set_points(0, initial)
     sim.synchronize()
     frames = 20
@@ -140,7 +237,13 @@ def main():
         sim.get_points(0)
     frame_s = (time.perf_counter() - tr) / frames
     with_readback = {"ms_per_frame": frame_s * 1e3, "sim_s_per_s": dt / frame_s,
-                     "readback_bytes_per_frame": 40 * N, "frames": frames}
+                     "readback_bytes_per_frame": 40 * N, "frames": frames, "math": best}
+    del sim
+
+    extra = {}
+    if not args.no_extra:
+        extra["batched"] = batched_plain(args, np, fa, wl)
+        extra["surfaces"] = surfaces_plain(args, np, fa, wl)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -163,111 +266,203 @@ def main():
             "vertex_steps_per_s": args.cpu_steps * N / cpu_wall,
         }
 
-    # the headline is the fastest form that meets the error bar (the plain step has no error)
-    use_sur = sur is not None and sur["rms_error_over_extent"] <= 1e-3 and sur["sim_s_per_s"] > sim_s_per_s
+    line = headline_line(args, legs, best, sur, compare, extra, N, dt, scene)
+    line["with_readback"] = with_readback
+    line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+def headline_line(args, legs, best, sur, compare, extra, N, dt, scene):
+    """The JSON line: the faster plain form, or the surrogate when it meets its bar and is faster."""
+    use_sur = sur is not None and sur["meets_bar"] and sur["sim_s_per_s"] > legs[best]["sim_s_per_s"]
+    L = legs[best]
+    roof = {"bound": "hbm", "achieved": L["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": L["hbm_frac"], "traffic": L["traffic"], "kernel": L["kernel"],
+            "kernel_avg_us": L["kernel_avg_us"], "bytes_per_launch": BYTES_PER_VERTEX_STEP * N}
+    if use_sur:
+        r = sur["mlp_roofline"]
+        roof = {"bound": "mfma", "achieved": r["achieved"], "peak": r["peak"], "unit": "TFLOP/s", "frac": r["frac"],
+                "traffic": None, "kernel": "sur_mlp_kernel", "kernel_avg_us": r["kernel_avg_us"],
+                "flop_per_launch": r["flop_per_launch"]}
     line = {
         "metric": "sim-s/s, 1M-vert cloth: surrogate vs plain @<=1e-3 pos err; %HBM peak",
-        "value": sur["sim_s_per_s"] if use_sur else sim_s_per_s,
-        "value_source": "surrogate" if use_sur else "plain",
-        "plain_sim_s_per_s": sim_s_per_s,
+        "value": sur["sim_s_per_s"] if use_sur else L["sim_s_per_s"],
+        "value_source": "surrogate" if use_sur else f"plain ({best} math)",
         "unit": "sim-s/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": L["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"plain springy step, {args.n}x{args.n} grid cloth ({N} vertices), top edge pinned, "
-                               f"320-face icosphere collider, RK4 dt=1/600 s",
-                   "vertices": N, "dt": dt, "kernel": args.kernel, "math": args.math},
-        "vertex_steps_per_s": args.steps * N / wall,
-        "fast_math": fast,
-        "surrogate": sur,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((args.n, args.math, args.kernel, args.tile)),
-                     "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": BYTES_PER_VERTEX_STEP * N},
-        "valu_issue": MEASURED_VALU.get((args.n, args.math, args.kernel, args.tile)),
-        "with_readback": with_readback,
-        "cpu_baseline": cpu,
+                               f"320-face icosphere collider, RK4 dt=1/600 s, {graph_chunk(args.steps, args.chunk)}"
+                               f"-step HIP graphs",
+                   "vertices": N, "dt": dt, "kernel": args.kernel, "math": best,
+                   "math_bars": {"exact": "bit-identical to the reference's f32 step",
+                                 "fast": "one-step forces and states within 1e-5 relative of the reference"}},
+        "vertex_steps_per_s": L["vertex_steps_per_s"],
+        "plain": legs,
+        "roofline": roof,
+        "valu_issue": L["valu_issue"],
     }
-    print(json.dumps(line), flush=True)
+    if compare is not None:
+        line["plain_600_step_comparison"] = compare
+    if sur is not None:
+        line["surrogate"] = sur
+    line.update(extra)
+    return line
 
 
 MLP_FLOP_PER_VERTEX = 2 * (72 * 128 + 128 * 128 + 128 * 128 + 128 * 3)  # 84,736
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def surrogate_leg(args, np, fa, wl, kern, ref, initial, N, dt, plain_sim_s_per_s):
-    """The same cloth stepped by the learned surrogate (8x steps, per-mesh fallback on) for the same
-    simulated time, timed on the host clock around the whole run (it reads each big step's
-    diagnostics back); the network kernel's launches timed with HIP events.  Error: the final
-    positions against the plain run's."""
+def surrogate_leg(args, np, fa, wl, sim, kern, ref_exact, initial, N, dt, legs):
+    """The same cloth stepped by the learned surrogate (8x steps, per-mesh fallback on) over the
+    full 600-step horizon whatever --steps is, timed on the host clock around the whole run (it
+    reads each big step's diagnostics back); the network kernel's launches timed with HIP events.
+    Error: the final positions against the exact plain run after the same number of dt (the plain
+    step of springy/simulation.rs:27-45 is the judge)."""
     if not args.weights or not os.path.exists(args.weights):
         return None
     params = np.load(args.weights, allow_pickle=False)
     net = fa.Surrogate(fa.SurrogateConfig(), params=params)
     ratio = net.config.ratio
-    n_big = args.steps // ratio
-    sim = wl.build_gpu(wl.config1(n=args.n, steps=args.steps))
-    sim.set_kernel(kern, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
+    n_big = SURROGATE_HORIZON // ratio
+    assert n_big * ratio == SURROGATE_HORIZON, "the surrogate must cover exactly the plain run's dt count"
+    sim.set_kernel(kern, fa.FP_MATH_FAST if "fast" in legs else fa.FP_MATH_EXACT)
     sim.set_surrogate(net)
     sim.set_surrogate_shape(args.sur_shape)
-    if args.warmup:
-        sim.step_surrogate(max(1, args.warmup // ratio))
-        sim.set_points(0, initial)
+    sim.set_points(0, initial)
+    sim.step_surrogate(1)  # warm-up (first launches), then the initial state again
+    sim.set_points(0, initial)
+    sim.set_surrogate(net)  # re-attaching clears the fallback backoff the warm-up may have left
+    sim.set_surrogate_shape(args.sur_shape)
     sim.synchronize()
     total_ms, mlp_ms, st = sim.time_surrogate(n_big)
+    assert st.big_steps == n_big
     x = sim.get_points(0)[:, 1:4].astype(np.float64)
-    xr = ref[:, 1:4].astype(np.float64)
     x0 = initial[:, 1:4].astype(np.float64)
     extent = (args.n - 1) * 0.05
-    e = np.linalg.norm(x - xr, axis=1)
-    disp = float(np.sqrt(np.mean(np.sum((xr - x0) ** 2, axis=1))))
+    e = np.linalg.norm(x - ref_exact, axis=1)
+    disp = float(np.sqrt(np.mean(np.sum((ref_exact - x0) ** 2, axis=1))))
+    rms = float(np.sqrt(np.mean(e * e)))
     sim_s = n_big * ratio * dt / (total_ms * 1e-3)
     launches = n_big - st.plain_only  # big steps that ran the network
     mlp_avg_s = mlp_ms * 1e-3 / max(1, launches)
-    tflops = MLP_FLOP_PER_VERTEX * N / mlp_avg_s / 1e12 if mlp_avg_s > 0 else None
+    tflops = MLP_FLOP_PER_VERTEX * N / mlp_avg_s / 1e12 if mlp_avg_s > 0 and launches > 0 else None
+    finite = bool(np.isfinite(x).all())
     return {
-        "sim_s_per_s": sim_s, "speedup_vs_plain": sim_s / plain_sim_s_per_s, "big_steps": n_big, "ratio": ratio,
+        "sim_s_per_s": sim_s,
+        "speedup_vs_plain": {m: sim_s / legs[m]["sim_s_per_s"] for m in legs},
+        "big_steps": n_big, "ratio": ratio, "plain_steps_covered": n_big * ratio,
         "ms_per_big_step": total_ms / max(1, n_big),
-        "rms_error_over_extent": float(np.sqrt(np.mean(e * e))) / extent,
+        "rms_error_over_extent": rms / extent,
         "max_error_over_extent": float(e.max()) / extent,
-        "rms_error_over_rms_displacement": float(np.sqrt(np.mean(e * e))) / disp if disp > 0 else None,
-        "finite": bool(np.isfinite(x).all()),
+        "rms_error_over_rms_displacement": rms / disp if disp > 0 else None,
+        "rms_displacement_m": disp,
+        "meets_bar": finite and rms / extent <= 1e-3,
+        "finite": finite,
         "fallbacks": st.fallbacks, "plain_only": st.plain_only, "network_launches": launches,
         "max_strain_residual": st.max_strain_residual,
         "max_energy_residual": st.max_energy_residual,
-        "error_definition": "final positions vs the plain dt=1/600 run after the same simulated time; "
-                            f"extent = cloth width {extent:.2f} m",
+        "error_definition": f"RMS over vertices of |x_surrogate - x_plain| after {n_big * ratio} dt, the plain run "
+                            f"being the exact step from the same initial state; extent = cloth width {extent:.2f} m",
         "mlp_roofline": {"bound": "mfma", "achieved": tflops, "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                          "frac": tflops / MFMA_BF16_PEAK_TFLOPS if tflops else None,
                          "kernel_avg_us": mlp_avg_s * 1e6, "flop_per_launch": MLP_FLOP_PER_VERTEX * N},
     }
 
 
-def bench_batched(args, np, fa, wl):
-    """4096 cloths (sides log-uniform in [8, 256], ~19.7M vertices), each with its own Config, one
-    tiled launch per tile shape per step, steps replayed from HIP graphs of `chunk` steps."""
+def only_mlp(args, np, fa, wl):
+    """The surrogate network kernel alone over the 1024 x 1024 cloth's vertices (features gathered
+    from the state, three GELU layers, the output), --steps launches; for rocprofv3."""
+    scene = wl.config1(n=args.n, steps=args.steps)
+    sim = wl.build_gpu(scene)
+    net = fa.Surrogate(fa.SurrogateConfig(), params=np.load(args.weights, allow_pickle=False))
+    sim.set_surrogate(net)
+    sim.set_surrogate_shape(args.sur_shape)
+    N = scene.cloths[0].rows * scene.cloths[0].cols
+    for _ in range(max(1, args.warmup // 10)):
+        sim.surrogate_accel(0)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        a = sim.surrogate_accel(0)
+    wall = time.perf_counter() - t
+    print(json.dumps({"metric": "surrogate network launches (profiling entry point)", "value": args.steps / wall,
+                      "unit": "launches/s (host clock, incl. readback)", "launches": args.steps, "vertices": N,
+                      "finite": bool(np.isfinite(a).all())}), flush=True)
+
+
+def plain_rates(sim, fa, initial, steps, warmup, chunk, n_vertices, dt, bytes_step):
+    out = {}
+    for m in ("exact", "fast"):
+        sim.set_kernel(fa.FP_KERNEL_AUTO, fa.FP_MATH_EXACT if m == "exact" else fa.FP_MATH_FAST)
+        wall, kernel_ms = time_plain(sim, initial, steps, warmup, chunk)
+        kavg = kernel_ms * 1e-3 / steps
+        out[m] = {"sim_s_per_s": steps * dt / wall, "ms_per_step": wall * 1e3 / steps,
+                  "vertex_steps_per_s": steps * n_vertices / wall, "kernel_avg_us": kavg * 1e6,
+                  "achieved_gbs": bytes_step / kavg / 1e9, "hbm_frac": bytes_step / kavg / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
+def batched_plain(args, np, fa, wl):
+    """Plain rates of the 4,096-cloth batch (per-cloth Config, tile shapes chosen by the autotuner)."""
     scene = wl.config2(steps=args.steps)
     sim = wl.build_gpu(scene)
-    sim.set_kernel(fa.FP_KERNEL_AUTO, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
+    N = sum(c.rows * c.cols for c in scene.cloths)
+    chosen = sim.autotune(10)
+    initial = [sim.get_points(k) for k in range(len(scene.cloths))]
+    chunk = graph_chunk(args.steps, args.chunk)
+    rates = plain_rates(sim, fa, initial, args.steps, args.warmup, chunk, N, scene.config.dt, BYTES_PER_VERTEX_STEP * N)
+    return {"workload": f"{len(scene.cloths)} grid cloths, sides log-uniform in [8, 256] ({N} vertices), per-cloth "
+                        f"stiffness/pins/wind, RK4 dt=1/600 s, {chunk}-step HIP graphs",
+            "vertices": N, "tile_shape_per_size_class": chosen, "steps": args.steps,
+            "bytes_per_vertex_step": BYTES_PER_VERTEX_STEP, "plain": rates}
+
+
+def surface_bytes(N, S, F):
+    # algorithmic bytes per step of a general mesh: the state read and written once (48 B/vertex),
+    # a row pointer (4 B/vertex), per directed strut entry the neighbour id and rest length (8 B),
+    # per face its 3 vertex ids and one constant (16 B)
+    return 48 * N + 4 * N + 8 * 2 * S + 16 * F
+
+
+def surfaces_plain(args, np, fa, wl):
+    """Plain rates of the 64-icosphere workload (patch kernel)."""
+    scene = wl.config4()
+    sim = wl.build_surface_gpu(scene)
+    N = sum(m.positions.shape[0] for m in scene.meshes)
+    S = sum(sim.mesh_counts(k)[1] for k in range(len(scene.meshes)))
+    F = sum(sim.mesh_counts(k)[2] for k in range(len(scene.meshes)))
+    initial = [sim.get_points(k) for k in range(len(scene.meshes))]
+    chunk = graph_chunk(args.steps, args.chunk)
+    bstep = surface_bytes(N, S, F)
+    rates = plain_rates(sim, fa, initial, args.steps, args.warmup, chunk, N, scene.config.dt, bstep)
+    rates["exact"]["traffic"] = traffic_of("patch_kernel<ExactMath>", "icospheres64")
+    return {"workload": f"64 radially perturbed icospheres (levels 4-7), {N} vertices, {S} struts, {F} faces, ground "
+                        f"plane, RK4 dt=1/600 s, {chunk}-step HIP graphs",
+            "vertices": N, "struts": S, "faces": F, "steps": args.steps, "bytes_per_step": bstep, "plain": rates}
+
+
+def bench_batched(args, np, fa, wl):
+    """4096 cloths (sides log-uniform in [8, 256], ~19.7M vertices), each with its own Config, one
+    tiled launch per tile shape per step, steps replayed from HIP graphs of `chunk` steps; then the
+    cloth surrogate on the same batch."""
+    scene = wl.config2(steps=args.steps)
+    sim = wl.build_gpu(scene)
+    math = "exact" if args.math != "fast" else "fast"
+    sim.set_kernel(fa.FP_KERNEL_AUTO, fa.FP_MATH_EXACT if math == "exact" else fa.FP_MATH_FAST)
     N = sum(c.rows * c.cols for c in scene.cloths)
     dt = scene.config.dt
     chosen = sim.autotune(10)
     initial = [sim.get_points(k) for k in range(len(scene.cloths))]
-    if args.warmup:
-        sim.step_graph(args.warmup, args.chunk)
-        for k, rec in enumerate(initial):
-            sim.set_points(k, rec)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    kernel_ms = sim.time_steps_graph(args.steps, args.chunk)
-    sim.synchronize()
-    wall = time.perf_counter() - t0
+    chunk = graph_chunk(args.steps, args.chunk)
+    wall, kernel_ms = time_plain(sim, initial, args.steps, args.warmup, chunk)
     kernel_avg_s = kernel_ms * 1e-3 / args.steps
     achieved = BYTES_PER_VERTEX_STEP * N / kernel_avg_s / 1e9
 
@@ -290,33 +485,34 @@ def bench_batched(args, np, fa, wl):
                          f"to the {N}-vertex batch (single-threaded C++ restatement of the reference step, one "
                          f"Simulation per cloth)", "vertex_steps_per_s": vsps}
     # the same cloths with the cloth surrogate network (8x steps, per-mesh fallback): a rejection
-    # costs only its own mesh's plain steps
+    # costs only its own mesh's plain steps; compared with the plain run over the same dt count
     sur = None
     if args.weights and os.path.exists(args.weights):
         M = len(scene.cloths)
-        ref = [sim.get_points(k)[:, 1:4].astype(np.float64) for k in range(M)]
         net = fa.Surrogate(fa.SurrogateConfig(), params=np.load(args.weights, allow_pickle=False))
-        ssim = wl.build_gpu(scene)
-        ssim.set_surrogate(net)
         ratio = net.config.ratio
-        n_big = args.steps // ratio
-        if args.warmup:
-            ssim.step_surrogate(max(1, args.warmup // ratio))
-            for k, rec in enumerate(initial):
-                ssim.set_points(k, rec)
-        total_ms, mlp_ms, st = ssim.time_surrogate(n_big)
+        n_big = SURROGATE_HORIZON // ratio
+        for k, rec in enumerate(initial):
+            sim.set_points(k, rec)
+        sim.step_graph(n_big * ratio, graph_chunk(n_big * ratio, args.chunk))
+        ref = [sim.get_points(k)[:, 1:4].astype(np.float64) for k in range(M)]
+        for k, rec in enumerate(initial):
+            sim.set_points(k, rec)
+        sim.set_surrogate(net)
+        total_ms, mlp_ms, st = sim.time_surrogate(n_big)
         rel, diverged = [], 0
         for k, c in enumerate(scene.cloths):
             if not (np.isfinite(ref[k]).all() and np.abs(ref[k]).max() < 1e3):
                 diverged += 1  # the plain method itself blows up on this cloth (e.g. corner pins)
                 continue
-            d = ssim.get_points(k)[:, 1:4].astype(np.float64) - ref[k]
+            d = sim.get_points(k)[:, 1:4].astype(np.float64) - ref[k]
             rel.append(float(np.sqrt(np.mean(np.sum(d * d, axis=1)))) / ((max(c.rows, c.cols) - 1) * c.spacing))
         rel = np.array(rel)
         rel = np.where(np.isfinite(rel), rel, np.inf)
         sim_s = n_big * ratio * dt / (total_ms * 1e-3)
         launches = n_big - st.plain_only
         sur = {"sim_s_per_s": sim_s, "speedup_vs_plain": sim_s / (args.steps * dt / wall), "big_steps": n_big,
+               "plain_steps_covered": n_big * ratio,
                "fallbacks": st.fallbacks, "mesh_fallbacks": st.mesh_fallbacks,
                "accepted_mesh_steps_frac": 1.0 - st.mesh_fallbacks / float(M * n_big),
                "plain_only": st.plain_only, "network_launches": launches,
@@ -325,7 +521,7 @@ def bench_batched(args, np, fa, wl):
                "cloths_within_1e-3": int(np.sum(rel <= 1e-3)), "cloths_compared": int(rel.size),
                "cloths_diverging_in_plain_method": diverged, "cloths": M,
                "error_definition": "per cloth: RMS of final-position differences vs the plain run after the same "
-                                   "simulated time, over the cloth's longer side",
+                                   "number of dt, over the cloth's longer side",
                "mlp_kernel_avg_us": mlp_ms * 1e3 / max(1, launches)}
     line = {
         "metric": "sim-s/s, batched cloths (4096 meshes, per-mesh Config); %HBM peak",
@@ -335,7 +531,7 @@ def bench_batched(args, np, fa, wl):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain springy step, {len(scene.cloths)} grid cloths, sides log-uniform in [8, 256] "
                                f"({N} vertices), per-cloth stiffness/pins/wind, RK4 dt=1/600 s, "
-                               f"{args.chunk}-step HIP graphs", "vertices": N, "math": args.math,
+                               f"{chunk}-step HIP graphs", "vertices": N, "math": math,
                    "tile_shape_per_size_class": chosen},
         "vertex_steps_per_s": args.steps * N / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -348,28 +544,20 @@ def bench_batched(args, np, fa, wl):
 
 
 def bench_surfaces(args, np, fa, wl):
-    """64 icospheres of uneven size (levels 4-7, 2,145,408 vertices) on the patch kernel."""
+    """64 icospheres of uneven size (levels 4-7, 2,145,408 vertices) on the patch kernel; then the
+    irregular-mesh surrogate on the same scene."""
     scene = wl.config4()
     sim = wl.build_surface_gpu(scene)
-    sim.set_kernel(fa.FP_KERNEL_AUTO, fa.FP_MATH_EXACT if args.math == "exact" else fa.FP_MATH_FAST)
+    math = "exact" if args.math != "fast" else "fast"
+    sim.set_kernel(fa.FP_KERNEL_AUTO, fa.FP_MATH_EXACT if math == "exact" else fa.FP_MATH_FAST)
     N = sum(m.positions.shape[0] for m in scene.meshes)
     S = sum(sim.mesh_counts(k)[1] for k in range(len(scene.meshes)))
     F = sum(sim.mesh_counts(k)[2] for k in range(len(scene.meshes)))
     dt = scene.config.dt
     initial = [sim.get_points(k) for k in range(len(scene.meshes))]
-    if args.warmup:
-        sim.step_graph(args.warmup, args.chunk)
-        for k, rec in enumerate(initial):
-            sim.set_points(k, rec)
-    sim.synchronize()
-    t0 = time.perf_counter()
-    kernel_ms = sim.time_steps_graph(args.steps, args.chunk)
-    sim.synchronize()
-    wall = time.perf_counter() - t0
-    # algorithmic bytes per step of a general mesh: the state read and written once (48 B/vertex),
-    # a row pointer (4 B/vertex), per directed strut entry the neighbour id and rest length (8 B),
-    # per face its 3 vertex ids and one constant (16 B)
-    bytes_step = 48 * N + 4 * N + 8 * 2 * S + 16 * F
+    chunk = graph_chunk(args.steps, args.chunk)
+    wall, kernel_ms = time_plain(sim, initial, args.steps, args.warmup, chunk)
+    bytes_step = surface_bytes(N, S, F)
     kernel_avg_s = kernel_ms * 1e-3 / args.steps
     achieved = bytes_step / kernel_avg_s / 1e9
     cpu = None
@@ -390,29 +578,30 @@ def bench_surfaces(args, np, fa, wl):
     sur = None
     wpath = os.path.join(ROOT, "feriphys_amd", "weights", "surface_surrogate.npy")
     if args.weights and os.path.exists(wpath):
-        # the same scene with the irregular-mesh surrogate (8x steps, per-mesh fallback), against the
-        # plain run's final positions (the simulation above ran exactly args.steps plain steps)
-        ref = [sim.get_points(k)[:, 1:4].astype(np.float64) for k in range(len(scene.meshes))]
+        # the irregular-mesh surrogate (8x steps, per-mesh fallback), against the plain run over the
+        # same number of dt
         net = fa.Surrogate(fa.SurrogateConfig(), params=np.load(wpath, allow_pickle=False))
-        ssim = wl.build_surface_gpu(scene)
-        ssim.set_surrogate(net)
-        n_big = args.steps // net.config.ratio
-        if args.warmup:
-            ssim.step_surrogate(max(1, args.warmup // net.config.ratio))
-            for k, rec in enumerate(initial):
-                ssim.set_points(k, rec)
-        total_ms, mlp_ms, st = ssim.time_surrogate(n_big)
+        n_big = SURROGATE_HORIZON // net.config.ratio
+        for k, rec in enumerate(initial):
+            sim.set_points(k, rec)
+        sim.step_graph(n_big * net.config.ratio, graph_chunk(n_big * net.config.ratio, args.chunk))
+        ref = [sim.get_points(k)[:, 1:4].astype(np.float64) for k in range(len(scene.meshes))]
+        for k, rec in enumerate(initial):
+            sim.set_points(k, rec)
+        sim.set_surrogate(net)
+        total_ms, mlp_ms, st = sim.time_surrogate(n_big)
         err2, ext = 0.0, max(float(np.ptp(m.positions, axis=0).max()) for m in scene.meshes)
         for k in range(len(scene.meshes)):
-            d = ssim.get_points(k)[:, 1:4].astype(np.float64) - ref[k]
+            d = sim.get_points(k)[:, 1:4].astype(np.float64) - ref[k]
             err2 += float(np.sum(d * d))
         launches = n_big - st.plain_only
         sur = {"sim_s_per_s": n_big * net.config.ratio * dt / (total_ms * 1e-3),
                "speedup_vs_plain": (n_big * net.config.ratio * dt / (total_ms * 1e-3)) / (args.steps * dt / wall),
-               "big_steps": n_big, "fallbacks": st.fallbacks, "mesh_fallbacks": st.mesh_fallbacks,
+               "big_steps": n_big, "plain_steps_covered": n_big * net.config.ratio,
+               "fallbacks": st.fallbacks, "mesh_fallbacks": st.mesh_fallbacks,
                "plain_only": st.plain_only, "network_launches": launches,
                "rms_error_over_extent": float(np.sqrt(err2 / N)) / ext,
-               "error_definition": "final positions vs the plain run after the same simulated time; extent = "
+               "error_definition": "final positions vs the plain run after the same number of dt; extent = "
                                    f"the largest mesh's diameter {ext:.3f} m",
                "mlp_kernel_avg_us": mlp_ms * 1e3 / max(1, launches)}
     line = {
@@ -421,10 +610,12 @@ def bench_surfaces(args, np, fa, wl):
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain springy step, 64 radially perturbed icospheres (levels 4-7), {N} vertices, "
-                               f"{S} struts, {F} faces, ground plane, RK4 dt=1/600 s", "math": args.math},
+                               f"{S} struts, {F} faces, ground plane, RK4 dt=1/600 s, {chunk}-step HIP graphs",
+                   "math": math},
         "vertex_steps_per_s": args.steps * N / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC_SURFACES.get(args.math),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_of("patch_kernel<ExactMath>", "icospheres64") if math == "exact" else None,
                      "kernel_avg_us": kernel_avg_s * 1e6, "bytes_per_launch": bytes_step},
         "cpu_baseline": cpu,
         "surrogate": sur,

@@ -89,6 +89,7 @@ SIGNATURES = {
     "fp_cloth_new_in_plane": (C.c_int, [_U64, _U64, C.c_float, _FP, C.c_float, C.c_float, C.c_float, C.c_float,
                                         C.c_float, C.c_float, C.c_float, _U64P, _U64, C.c_int32, _PP]),
     "fp_mesh_add_strut": (C.c_int, [_P, _U64, _U64, C.c_float, C.c_float]),
+    "fp_mesh_add_struts": (C.c_int, [_P, _U64P, _U64, C.c_float, C.c_float]),
     "fp_mesh_add_pin": (C.c_int, [_P, _U64]),
     "fp_mesh_counts": (C.c_int, [_P, _U64P, _U64P, _U64P, _U64P]),
     "fp_mesh_get_struts": (C.c_int, [_P, _U64P, _FP, _I64P]),

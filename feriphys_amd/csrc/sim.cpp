@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "grid_fast.hpp"
 #include "sim_internal.hpp"
 
 
@@ -179,7 +180,11 @@ void refresh_mass(MeshSlot& ms, const float* rec, uint64_t n) {
 int upload_params(fp_sim* s) {
   std::vector<fp::StepParams> p;
   for (auto& m : s->meshes) p.push_back(m.params);
+  if (s->params.p != nullptr && p.size() == s->h_params_last.size() &&
+      std::memcmp(p.data(), s->h_params_last.data(), p.size() * sizeof(fp::StepParams)) == 0)
+    return FP_OK;  // the device copy already holds these values
   HIP_TRY(s->params.assign(p.data(), p.size()));
+  s->h_params_last = p;
   ++s->epoch;
   return FP_OK;
 }
@@ -207,6 +212,14 @@ std::vector<T> xcd_order(const std::vector<T>& t) {
 }
 
 int build_block_lists(fp_sim* s) {
+  // the lists depend on the kernel selection and, per mesh, on whether its masses are uniform and on
+  // its tile shape (topology, blocks and patches are fixed when the simulation is built)
+  std::vector<int32_t> sig{s->kernel, s->math};
+  for (const auto& m : s->meshes) {
+    sig.push_back(m.uniform_mass ? 1 : 0);
+    sig.push_back(m.tile_shape);
+  }
+  if (sig == s->lists_sig) return FP_OK;
   std::vector<fp::BlockRange> g, c, cf, sb, sf;
   std::vector<fp::PatchRange> pl;
   std::vector<fp::TileRange> by_shape[fp::N_TILE_SHAPES], all_tiles;
@@ -219,9 +232,11 @@ int build_block_lists(fp_sim* s) {
     ms.tile_begin = ms.tile_end = (uint32_t)all_tiles.size();
     ms.on_patch = false;
     if (ms.kernel == MK_GRID_TILED) {
+      // the fast kernel has one quad per thread: 15x15 and 7x31 tiles (16x16 becomes 15x15)
+      ms.launch_shape = s->math == FP_MATH_FAST && ms.tile_shape == fp::TILE_16x16 ? fp::TILE_15x15 : ms.tile_shape;
       std::vector<fp::TileRange> mt;
-      cloth_tiles(mi, ms.host.cloth, ms.tile_shape, mt);
-      by_shape[ms.tile_shape].insert(by_shape[ms.tile_shape].end(), mt.begin(), mt.end());
+      cloth_tiles(mi, ms.host.cloth, ms.launch_shape, mt);
+      by_shape[ms.launch_shape].insert(by_shape[ms.launch_shape].end(), mt.begin(), mt.end());
       all_tiles.insert(all_tiles.end(), mt.begin(), mt.end());
       ms.tile_end = (uint32_t)all_tiles.size();
       continue;
@@ -267,6 +282,7 @@ int build_block_lists(fp_sim* s) {
     HIP_TRY(s->tiles[sh].upload(o.data(), o.size()));
   }
   HIP_TRY(s->mesh_tiles.upload(all_tiles.data(), all_tiles.size()));
+  s->lists_sig = sig;
   ++s->epoch;
   return FP_OK;
 }
@@ -283,13 +299,20 @@ int check_sim_mesh(const fp_sim* s, uint64_t mesh) {
   return FP_OK;
 }
 
+// The plain step (or the forces) of cloths on the tiled kernels: the exact kernel, or for the fast
+// math form the fast kernel (which has no 16x16 shape: its tiles are built as 15x15).
+hipError_t tiled_launch(fp_sim* s, const fp::StepArgs& a, const fp::TileRange* tiles, uint32_t n, int shape, int mode) {
+  if (s->math == FP_MATH_FAST && shape != fp::TILE_16x16)
+    return fp::launch_grid_fast(a, s->cloths.p, tiles, n, shape, mode, s->stream);
+  return fp::launch_grid_tiled(a, s->cloths.p, tiles, n, shape, mode, s->math == FP_MATH_EXACT, s->stream);
+}
+
 int run_steps(fp_sim* s, uint64_t n) {
   for (uint64_t k = 0; k < n; ++k) {
     fp::StepArgs a = s->args(s->forces_dirty);
     bool exact = s->math == FP_MATH_EXACT;
     for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh)
-      HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->tiles[sh].p, s->n_tiles[sh], sh, fp::MODE_STEP, exact,
-                                    s->stream));
+      HIP_TRY(tiled_launch(s, a, s->tiles[sh].p, s->n_tiles[sh], sh, fp::MODE_STEP));
     HIP_TRY(fp::launch_grid(a, s->cloths.p, s->grid_blocks.p, s->n_grid_blocks, fp::MODE_STEP, exact, s->stream));
     HIP_TRY(fp::launch_faces(a, s->csr(), s->csr_face_blocks.p, s->n_csr_face_blocks, exact, s->stream));
     if (s->torsion_on)
@@ -664,6 +687,18 @@ int fp_mesh_add_strut(fp_mesh* mesh, uint64_t v0, uint64_t v1, float k, float c)
   if (int e = check_mesh(mesh)) return e;
   fp::Status st = fp::mesh_add_strut(mesh->m, v0, v1, k, c);
   return st.code == FP_OK ? FP_OK : fail(st);
+}
+
+int fp_mesh_add_struts(fp_mesh* mesh, const uint64_t* v01, uint64_t n, float k, float c) {
+  if (int e = check_mesh(mesh)) return e;
+  if (n && !v01) return fail(FP_EINVAL, "strut pairs are NULL");
+  for (uint64_t s = 0; s < 2 * n; ++s)  // all or nothing: check every index before appending
+    if (v01[s] >= mesh->m.n) return fail(FP_EINVAL, "add_strut: vertex index out of range");
+  for (uint64_t s = 0; s < n; ++s) {
+    fp::Status st = fp::mesh_add_strut(mesh->m, v01[2 * s], v01[2 * s + 1], k, c);
+    if (st.code != FP_OK) return fail(st);
+  }
+  return FP_OK;
 }
 
 int fp_mesh_add_pin(fp_mesh* mesh, uint64_t index) {
@@ -1115,8 +1150,8 @@ int fp_sim_accumulate_forces(fp_sim* s, uint64_t mesh) {
   uint32_t nb = m.blk_end - m.blk_begin;
   bool exact = s->math == FP_MATH_EXACT;
   if (m.kernel == MK_GRID_TILED)
-    HIP_TRY(fp::launch_grid_tiled(a, s->cloths.p, s->mesh_tiles.p + m.tile_begin, m.tile_end - m.tile_begin,
-                                  m.tile_shape, fp::MODE_FORCES, exact, s->stream));
+    HIP_TRY(tiled_launch(s, a, s->mesh_tiles.p + m.tile_begin, m.tile_end - m.tile_begin, m.launch_shape,
+                         fp::MODE_FORCES));
   else if (m.kernel == MK_GRID_SIMPLE) HIP_TRY(fp::launch_grid(a, s->cloths.p, blk, nb, fp::MODE_FORCES, exact, s->stream));
   else {
     HIP_TRY(fp::launch_faces(a, s->csr(), s->all_face_blocks.p + m.fblk_begin, m.fblk_end - m.fblk_begin, exact,

@@ -31,6 +31,7 @@ struct MeshSlot {
   bool uniform_mass = true;
   int kernel = MK_CSR;
   int tile_shape = fp::TILE_16x16;
+  int launch_shape = fp::TILE_16x16;  // the shape its tiles are built with under the current math form
   uint32_t blk_begin = 0, blk_end = 0;
   uint32_t fblk_begin = 0, fblk_end = 0;  // face blocks (range in fp_sim::h_face_blocks)
   uint32_t tile_begin = 0, tile_end = 0;  // range in fp_sim::mesh_tiles
@@ -100,6 +101,10 @@ struct fp_sim {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // captured step chunks, one per state-buffer parity; valid while `epoch` is unchanged
   uint64_t epoch = 1;
+  // what the last upload_params / build_block_lists derived their device lists from: a call that
+  // would upload the same lists again does nothing (so a checkpoint restore keeps the graphs)
+  std::vector<fp::StepParams> h_params_last;
+  std::vector<int32_t> lists_sig;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   uint64_t graph_epoch[2] = {0, 0};
   uint32_t graph_chunk[2] = {0, 0};

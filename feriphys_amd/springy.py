@@ -126,6 +126,12 @@ class SpringyMesh(_Handle):
     def add_strut(self, vertex_indices: Sequence[int], stiffness: float, damping: float):
         check(lib().fp_mesh_add_strut(self.handle, int(vertex_indices[0]), int(vertex_indices[1]), stiffness, damping))
 
+    def add_struts(self, pairs, stiffness: float, damping: float):
+        """add_strut for each (v0, v1) row of `pairs`, in order, in one call (no reference
+        counterpart; the same struts as the loop)."""
+        v01 = np.ascontiguousarray(np.asarray(pairs, np.uint64).reshape(-1, 2))
+        check(lib().fp_mesh_add_struts(self.handle, _uptr(v01), v01.shape[0], stiffness, damping))
+
     def add_pin(self, pin_index: int):
         check(lib().fp_mesh_add_pin(self.handle, int(pin_index)))
 

@@ -410,8 +410,7 @@ def build_surface_gpu(scene: "SurfaceScene", threads: int = 8):
         m = sp.SpringyMesh(spec.positions, spec.indices, spec.mass, spec.k, spec.c)
         v01, _, faces = m.struts()
         _, fv = m.faces()
-        for a, b in bend_pairs(v01, faces, fv):
-            m.add_strut((int(a), int(b)), spec.bk, spec.bc)
+        m.add_struts(bend_pairs(v01, faces, fv).astype(np.uint64), spec.bk, spec.bc)
         return m
 
     with ThreadPoolExecutor(max_workers=threads) as ex:

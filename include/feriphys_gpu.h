@@ -114,6 +114,10 @@ int fp_cloth_new_in_plane(uint64_t rows, uint64_t cols, float spacing, const flo
 
 /* SpringyMesh::add_strut (springy_mesh.rs:403-413): appended, no de-duplication. */
 int fp_mesh_add_strut(fp_mesh* mesh, uint64_t v0, uint64_t v1, float stiffness, float damping);
+/* No counterpart in the reference: n calls of SpringyMesh::add_strut (springy_mesh.rs:403-413) in
+ * order, pairs v01[2 s], v01[2 s + 1], all with the same stiffness and damping.  Every index is
+ * checked first; on FP_EINVAL nothing is appended. */
+int fp_mesh_add_struts(fp_mesh* mesh, const uint64_t* v01, uint64_t n, float stiffness, float damping);
 /* SpringyMesh::add_pin (springy_mesh.rs:415-417): appended, duplicates kept.  The index is
  * checked when the mesh is moved into a simulation (the reference panics at first use). */
 int fp_mesh_add_pin(fp_mesh* mesh, uint64_t index);

@@ -1,0 +1,96 @@
+// CPU emulation of the fast-math tiled grid kernel (feriphys_amd/csrc/grid_fast_core.hpp): every
+// tile's three phases run on the host in barrier order, with each LDS index bounds-checked and the
+// LDS filled with NaN beforehand, so a read of a slot that no phase wrote shows in the results.
+// Test infrastructure only: it lets the tests compare the kernel's tile logic with the oracle
+// without a GPU.  No obstacles (the collision functions are the exact kernels' own).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#define FP_LDS_CHECK(cond)                                                      \
+  do {                                                                          \
+    if (!(cond)) {                                                              \
+      std::fprintf(stderr, "fast_emul: index check failed: %s\n", #cond);       \
+      std::abort();                                                             \
+    }                                                                           \
+  } while (0)
+#include "../../feriphys_amd/csrc/grid_fast_core.hpp"
+
+namespace {
+
+template <int TR, int TC, int MODE>
+void run(const fp::StepArgs& A, const fp::ClothView& G, const fp::StepParams& P) {
+  using L_t = fp::gfast::Lds<TR, TC>;
+  std::unique_ptr<L_t> L(new L_t);
+  const int nthreads = (int)fp::kBlock;
+  for (uint32_t r0 = 0; r0 < G.rows; r0 += TR)
+    for (uint32_t c0 = 0; c0 < G.cols; c0 += TC) {
+      std::memset((void*)L.get(), 0xff, sizeof(L_t));  // NaN everywhere
+      for (int t = 0; t < nthreads; ++t) fp::gfast::phase0<TR, TC>(*L, A, G, (int)r0, (int)c0, t, nthreads);
+      for (int t = 0; t < nthreads; ++t) fp::gfast::phase1<TR, TC>(*L, P, G, (int)r0, (int)c0, t);
+      for (int t = 0; t < nthreads; ++t)
+        fp::gfast::phase2<TR, TC, MODE>(*L, A, P, G, (int)r0, (int)c0, t,
+                                         [](fp::f3, fp::f3, fp::f3, fp::f3&, fp::f3&) {});
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// state: 6 planes of n floats (x, y, z, vx, vy, vz); out: the same after the step (mode 0); forces:
+// 3 planes, read when force_init and written in mode 1 (the forces) or cleared (mode 0 with
+// force_init); pins: bit mask over the vertices (may be NULL); cfg: integration, dt, dt_d, gravity[3],
+// wind[3], drag, lift, restitution, friction, mass; shape: 0 = 15x15 tiles, 1 = 7x31.
+int emul_fast(uint32_t rows, uint32_t cols, float spacing, const float* position, const float* kc6, const float* state,
+              float* out, float* forces, int32_t force_init, const uint32_t* pins, int32_t integration,
+              const float* cfg, int32_t mode, int32_t shape) {
+  const uint32_t n = rows * cols;
+  fp::ClothView G{};
+  G.base = 0;
+  G.rows = rows;
+  G.cols = cols;
+  G.spacing = spacing;
+  G.cx = (float)cols / 2.0f * spacing;
+  G.cy = (float)rows / 2.0f * spacing;
+  G.p0 = position[0];
+  G.p1 = position[1];
+  G.tk = kc6[0]; G.tc = kc6[1]; G.sk = kc6[2]; G.sc = kc6[3]; G.bk = kc6[4]; G.bc = kc6[5];
+  fp::StepParams P{};
+  P.integration = integration;
+  P.dt = cfg[0];
+  P.dt_half = cfg[0] * 0.5f;
+  P.dt6 = cfg[0] / 6.0f;
+  P.dt3 = cfg[0] / 3.0f;
+  P.dt_d = cfg[1];
+  P.gravity = fp::f3{cfg[2], cfg[3], cfg[4]};
+  P.wind = fp::f3{cfg[5], cfg[6], cfg[7]};
+  P.neg_drag = -1.0f * cfg[8];
+  P.neg_lift = -1.0f * cfg[9];
+  P.restitution = cfg[10];
+  P.friction = cfg[11];
+  P.mass = cfg[12];
+  P.uniform_mass = 1;
+  std::vector<float> st(state, state + 6 * (size_t)n);
+  fp::StepArgs A{};
+  A.in = fp::SoA{st.data(), st.data() + n, st.data() + 2 * n, st.data() + 3 * n, st.data() + 4 * n, st.data() + 5 * n};
+  A.out = fp::SoA{out, out + n, out + 2 * n, out + 3 * n, out + 4 * n, out + 5 * n};
+  A.fx = forces;
+  A.fy = forces + n;
+  A.fz = forces + 2 * n;
+  A.pins = pins;
+  A.force_init = force_init;
+  A.obs = fp::ObsView{};
+  if (shape == 0) {
+    if (mode == 0) run<15, 15, fp::MODE_STEP>(A, G, P);
+    else run<15, 15, fp::MODE_FORCES>(A, G, P);
+  } else {
+    if (mode == 0) run<7, 31, fp::MODE_STEP>(A, G, P);
+    else run<7, 31, fp::MODE_FORCES>(A, G, P);
+  }
+  return 0;
+}
+
+}  // extern "C"

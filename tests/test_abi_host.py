@@ -313,3 +313,22 @@ def test_host_code_under_address_sanitizer(tmp_path):
                PYTHONPATH=ROOT + os.pathsep + os.environ.get("PYTHONPATH", ""))
     r = subprocess.run([sys.executable, "-c", _ASAN_SCRIPT, lib], capture_output=True, text=True, env=env, timeout=600)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.stdout[-2000:], r.stderr[-4000:])
+
+
+def test_add_struts_is_the_add_strut_loop():
+    """fp_mesh_add_struts appends the same struts as add_strut called once per pair (order, rest
+    lengths, constants), and on a bad index appends nothing."""
+    v, f = wl.icosphere_fast(2)
+    pairs = np.array([[0, 5], [7, 3], [11, 40], [2, 2]], np.uint64)
+    a = fa.SpringyMesh(v, f, 1.0, 10.0, 0.5)
+    b = fa.SpringyMesh(v, f, 1.0, 10.0, 0.5)
+    for p in pairs:
+        a.add_strut(p, 3.0, 0.25)
+    b.add_struts(pairs, 3.0, 0.25)
+    sa, sb = a.struts(), b.struts()
+    for x, y in zip(sa, sb):
+        assert (np.asarray(x) == np.asarray(y)).all()
+    n0 = b.counts()[1]
+    with pytest.raises(Exception):
+        b.add_struts(np.array([[0, 1], [0, len(v)]], np.uint64), 1.0, 1.0)
+    assert b.counts()[1] == n0
