@@ -152,9 +152,10 @@ def main():
                     help="cloth: one 1024x1024 cloth + sphere (default); batched: 4096 cloths of uneven sizes; "
                          "surfaces: 64 perturbed icospheres (2.1M vertices) on a ground plane; rigid: 1M unit "
                          "cubes tumbling onto a ground plane")
-    ap.add_argument("--only", choices=["all", "plain", "mlp"], default="all",
+    ap.add_argument("--only", choices=["all", "plain", "mlp", "copy"], default="all",
                     help="plain: only the timed plain steps of the cloth (for profiling the step kernel); "
-                         "mlp: only the surrogate network kernel over the cloth's vertices, --steps launches")
+                         "mlp: only the surrogate network kernel over the cloth's vertices, --steps launches; "
+                         "copy: --steps device copies of 256 MiB (the calibration of the HBM byte counters)")
     ap.add_argument("--no-extra", action="store_true", help="leave the batched and surfaces keys out of the line")
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph")
     ap.add_argument("--weights", default=os.path.join(ROOT, "feriphys_amd", "weights", "cloth_surrogate.npy"),
@@ -180,6 +181,8 @@ def main():
         return bench_rigid(args, np, fa, wl)
     if args.only == "mlp":
         return only_mlp(args, np, fa, wl)
+    if args.only == "copy":
+        return only_copy(args)
 
     scene = wl.config1(n=args.n, steps=args.steps)
     sim = wl.build_gpu(scene)
@@ -396,6 +399,25 @@ def only_mlp(args, np, fa, wl):
     print(json.dumps({"metric": "surrogate network launches (profiling entry point)", "value": args.steps / wall,
                       "unit": "launches/s (host clock, incl. readback)", "launches": args.steps, "vertices": N,
                       "finite": bool(np.isfinite(a).all())}), flush=True)
+
+
+def only_copy(args):
+    """A plain device copy of 256 MiB, --steps times: what FETCH_SIZE and WRITE_SIZE read for a
+    kernel that moves a known number of bytes (each reads 256 MiB once and writes it once)."""
+    import torch
+
+    n = 256 * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t
+    print(json.dumps({"metric": "device copy (counter calibration entry point)", "bytes_read_per_copy": 4 * n,
+                      "bytes_written_per_copy": 4 * n, "copies": args.steps,
+                      "value": 2 * 4 * n * args.steps / wall / 1e9, "unit": "GB/s (host clock)"}), flush=True)
 
 
 def plain_rates(sim, fa, initial, steps, warmup, chunk, n_vertices, dt, bytes_step):
