@@ -10,28 +10,45 @@ namespace {
 
 template <int TR, int TC, int MODE>
 __global__ void __launch_bounds__(kBlock) grid_fast_kernel(StepArgs A, const ClothView* __restrict__ cloths,
-                                                           const TileRange* __restrict__ tiles) {
+                                                           const TileRange* __restrict__ tiles, HitBox hb) {
   static_assert((TR + 1) * (TC + 1) == (int)kBlock, "one quad per thread");
   __shared__ gfast::Lds<TR, TC> L;
   const TileRange T = tiles[blockIdx.x];
   const ClothView G = cloths[T.mesh];
   const StepParams P = A.params[T.mesh];
   const int r0 = (int)T.r0, c0 = (int)T.c0, tid = (int)threadIdx.x;
-  gfast::phase0<TR, TC>(L, A, G, r0, c0, tid, (int)kBlock);
-  __syncthreads();
-  gfast::phase1<TR, TC>(L, P, G, r0, c0, tid);
-  __syncthreads();
-  gfast::phase2<TR, TC, MODE>(L, A, P, G, r0, c0, tid, [&](f3 p, f3 v, f3 F, f3& xn, f3& vn) {
-    const int32_t hit = first_hit(A.obs, p, xn, P.dt_d);
-    if (hit >= 0) collide_response(A.obs.faces[hit], P, P.mass, p, v, F, xn, vn);
-  });
+  // the obstacle response of update_points for segments that can reach an obstacle face
+  auto collide = [&](f3 p, f3 v, f3 F, f3& xn, f3& vn) {
+    const float m = A.obs.margin;
+    const bool near = fminf(p.x, xn.x) - m <= hb.hi[0] && fmaxf(p.x, xn.x) + m >= hb.lo[0] &&
+                      fminf(p.y, xn.y) - m <= hb.hi[1] && fmaxf(p.y, xn.y) + m >= hb.lo[1] &&
+                      fminf(p.z, xn.z) - m <= hb.hi[2] && fmaxf(p.z, xn.z) + m >= hb.lo[2];
+    if (near) {
+      const int32_t hit = first_hit(A.obs, p, xn, P.dt_d);
+      if (hit >= 0) collide_response(A.obs.faces[hit], P, P.mass, p, v, F, xn, vn);
+    }
+  };
+  if (gfast::interior_tile<TR, TC>(G, r0, c0)) {
+    gfast::phase0<TR, TC, true>(L, A, G, r0, c0, tid, (int)kBlock);
+    __syncthreads();
+    gfast::phase1<TR, TC, true>(L, P, G, r0, c0, tid);
+    __syncthreads();
+    gfast::phase2<TR, TC, MODE, true>(L, A, P, G, r0, c0, tid, collide);
+  } else {
+    gfast::phase0<TR, TC, false>(L, A, G, r0, c0, tid, (int)kBlock);
+    __syncthreads();
+    gfast::phase1<TR, TC, false>(L, P, G, r0, c0, tid);
+    __syncthreads();
+    gfast::phase2<TR, TC, MODE, false>(L, A, P, G, r0, c0, tid, collide);
+  }
 }
 
 template <int TR, int TC>
 hipError_t launch_shape(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n, int mode,
-                        hipStream_t s) {
-  if (mode == MODE_STEP) grid_fast_kernel<TR, TC, MODE_STEP><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
-  else if (mode == MODE_FORCES) grid_fast_kernel<TR, TC, MODE_FORCES><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles);
+                        const HitBox& hb, hipStream_t s) {
+  if (mode == MODE_STEP) grid_fast_kernel<TR, TC, MODE_STEP><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles, hb);
+  else if (mode == MODE_FORCES)
+    grid_fast_kernel<TR, TC, MODE_FORCES><<<dim3(n), dim3(kBlock), 0, s>>>(a, cloths, tiles, hb);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -39,11 +56,11 @@ hipError_t launch_shape(const StepArgs& a, const ClothView* cloths, const TileRa
 }  // namespace
 
 hipError_t launch_grid_fast(const StepArgs& a, const ClothView* cloths, const TileRange* tiles, uint32_t n_tiles,
-                            int shape, int mode, hipStream_t s) {
+                            int shape, int mode, const HitBox& box, hipStream_t s) {
   if (n_tiles == 0) return hipSuccess;
   switch (shape) {
-    case TILE_15x15: return launch_shape<15, 15>(a, cloths, tiles, n_tiles, mode, s);
-    case TILE_7x31: return launch_shape<7, 31>(a, cloths, tiles, n_tiles, mode, s);
+    case TILE_15x15: return launch_shape<15, 15>(a, cloths, tiles, n_tiles, mode, box, s);
+    case TILE_7x31: return launch_shape<7, 31>(a, cloths, tiles, n_tiles, mode, box, s);
     default: return hipErrorInvalidValue;
   }
 }

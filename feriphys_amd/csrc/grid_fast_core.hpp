@@ -22,6 +22,11 @@
 // tile vertex the six vectors addressed to it.  Summation order is fixed, so runs are
 // deterministic; it is not the reference's order (the exact kernel keeps that order).
 //
+// Tiles whose staged border lies inside the grid (all but the cloth's outer ring of tiles) take an
+// INTERIOR instantiation with no bounds checks and no validity selects: every strut and face their
+// threads evaluate exists, except thread (0, 0)'s two substituted bend struts, whose results land
+// only in slots that no vertex of the tile reads.
+//
 // Arithmetic: a strut's length |d| is the reference's (unfused dot, correctly rounded square root:
 // |d| - L cancels to the strain, and one ulp of |d| is already k ulp(L) of force); the rest uses
 // fused multiply-adds and the hardware reciprocal, reciprocal square root and square root; corner
@@ -59,6 +64,21 @@ FP_HD float sqrt_f(float x) {
   return sqrtf(x);
 #endif
 }
+// The correctly rounded square root for every input but subnormals (where it is within one ulp):
+// the hardware square root and the choice among it and its two neighbours by the exact residuals
+// (sqrt_rn of exact_ops.hpp without its range check, which only subnormal and huge inputs need;
+// zero, infinity and NaN come out right through the same selects).
+FP_HD float sqrt_cr(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float dn = __uint_as_float(__float_as_uint(s) - 1u), up = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rdn = __builtin_fmaf(-dn, s, x), rup = __builtin_fmaf(-up, s, x);
+  const float t = rdn <= 0.0f ? dn : s;
+  return rup > 0.0f ? up : t;
+#else
+  return sqrtf(x);
+#endif
+}
 FP_HD float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 FP_HD f3 fsub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 FP_HD float fdot(f3 a, f3 b) { return fma_(a.z, b.z, fma_(a.y, b.y, a.x * b.x)); }
@@ -89,7 +109,7 @@ FP_HD float corner_w(float cm, float d) {
 // damping c ((v1 - v0) . u) along u = d / |d|, d = p1 - p0.
 FP_HD f3 strut_f(f3 p0, f3 v0, f3 p1, f3 v1, float k, float c, float L) {
   const f3 d = fsub(p1, p0);
-  const float len = sqrt_rn((d.x * d.x + d.y * d.y) + d.z * d.z);
+  const float len = sqrt_cr((d.x * d.x + d.y * d.y) + d.z * d.z);
   const float inv = rcp_f(len);
   const float s = k * (len - L);
   const float dvd = fdot(fsub(v1, v0), d);
@@ -119,57 +139,60 @@ struct Lds {
   static constexpr int SH = TR + 4, SW = TC + 4;  // staged rows [r0-2, r0+TR+2), cols [c0-2, c0+TC+2)
   static constexpr int QR = TR + 1, QC = TC + 1;  // quads = threads
   static constexpr int NQ = QR * QC;
+  // strut tables, one array: row struts by first column c0-1+j (at H0), column struts by first row
+  // r0-1+i (V0), row bends by first column c0-2+j (BH0), column bends by first row r0-2+i (BV0)
+  static constexpr int H0 = 0, V0 = QC, BH0 = QC + QR, BV0 = 2 * QC + QR + 1, NTAB = 2 * (QC + QR) + 2;
   float4 pv[SH * SW];  // x, y, z, vx
   float2 vw[SH * SW];  // vy, vz
-  KTab h[QC], v[QR], bh[QC + 1], bv[QR + 1];
+  KTab tab[NTAB];
   float out[6][3][NQ];  // per quad thread: to_A, to_B, to_C, to_D, bend far ends (row, column)
 };
+
+// Whether a tile's staged region (the tile and a 2-vertex border) lies inside its grid.
+template <int TR, int TC>
+FP_HD bool interior_tile(const ClothView& G, int r0, int c0) {
+  return r0 >= 2 && c0 >= 2 && r0 + TR + 2 <= (int)G.rows && c0 + TC + 2 <= (int)G.cols;
+}
 
 FP_HD float grid_x(const ClothView& G, int c) { return ((float)c * G.spacing - G.cx) + G.p0; }
 FP_HD float grid_y(const ClothView& G, int r) { return ((float)r * G.spacing - G.cy) + G.p1; }
 
+// A strut along one grid axis: the reference's rest length sqrt(d * d + 0 * 0) is |d| exactly (a
+// correctly rounded square root of a correctly rounded square returns |d| in binary floating point).
 FP_HD KTab ktab(float d, float kn, float cn) {
-  const float L = sqrt_rn(d * d + 0.0f * 0.0f);
-  const float inv = rcp_rn(L);
+  const float L = fabsf(d);
+  const float inv = rcp_f(L);
   return KTab{d, L, inv * kn, inv * cn};
 }
 
 // Phase 0: stage the state of the tile and its 2-vertex border; the strut constant tables.
-template <int TR, int TC>
+template <int TR, int TC, bool INTERIOR>
 FP_HD void phase0(Lds<TR, TC>& L, const StepArgs& A, const ClothView& G, int r0, int c0, int tid, int nthreads) {
   using S = Lds<TR, TC>;
   const int R = (int)G.rows, C = (int)G.cols;
   for (int e = tid; e < S::SH * S::SW; e += nthreads) {
     const int lr = e / S::SW, lc = e - lr * S::SW;
     const int r = r0 - 2 + lr, c = c0 - 2 + lc;
-    if (r < 0 || r >= R || c < 0 || c >= C) continue;
+    if (!INTERIOR && (r < 0 || r >= R || c < 0 || c >= C)) continue;
     const uint32_t g = G.base + (uint32_t)(r * C + c);
     FP_LDS_CHECK(e >= 0 && e < S::SH * S::SW);
     L.pv[e] = float4{A.in.x[g], A.in.y[g], A.in.z[g], A.in.vx[g]};
     L.vw[e] = float2{A.in.vy[g], A.in.vz[g]};
   }
-  // tables: row struts by first column c0-1+j, column struts by first row r0-1+i, bends by first
-  // column c0-2+j and first row r0-2+i
-  for (int e = tid; e < 2 * (S::QC + S::QR) + 2; e += nthreads) {
-    if (e < S::QC) {
-      const int c = c0 - 1 + e;
-      L.h[e] = ktab(grid_x(G, c) - grid_x(G, c + 1), G.tk, G.tc);
-    } else if (e < S::QC + S::QR) {
-      const int q = e - S::QC, r = r0 - 1 + q;
-      L.v[q] = ktab(grid_y(G, r) - grid_y(G, r + 1), G.tk, G.tc);
-    } else if (e < 2 * S::QC + S::QR + 1) {
-      const int q = e - S::QC - S::QR, c = c0 - 2 + q;
-      L.bh[q] = ktab(grid_x(G, c) - grid_x(G, c + 2), G.bk, G.bc);
-    } else {
-      const int q = e - 2 * S::QC - S::QR - 1, r = r0 - 2 + q;
-      FP_LDS_CHECK(q >= 0 && q < S::QR + 1);
-      L.bv[q] = ktab(grid_y(G, r) - grid_y(G, r + 2), G.bk, G.bc);
-    }
+  for (int e = tid; e < S::NTAB; e += nthreads) {
+    // which table: coordinate axis (columns or rows), first index, reach (1 or 2), nominal constants
+    const bool bend = e >= S::BH0;
+    const int q = e - (e >= S::BV0 ? S::BV0 : e >= S::BH0 ? S::BH0 : e >= S::V0 ? S::V0 : S::H0);
+    const bool cols = e < S::V0 || (e >= S::BH0 && e < S::BV0);
+    const int first = (cols ? c0 : r0) - (bend ? 2 : 1) + q, reach = bend ? 2 : 1;
+    const float d = cols ? grid_x(G, first) - grid_x(G, first + reach) : grid_y(G, first) - grid_y(G, first + reach);
+    FP_LDS_CHECK(e >= 0 && e < S::NTAB);
+    L.tab[e] = ktab(d, bend ? G.bk : G.tk, bend ? G.bc : G.tc);
   }
 }
 
 // Phase 1: thread t = i * QC + j evaluates quad (r0-1+i, c0-1+j)'s struts and faces.
-template <int TR, int TC>
+template <int TR, int TC, bool INTERIOR>
 FP_HD void phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r0, int c0, int t) {
   using S = Lds<TR, TC>;
   if (t >= S::NQ) return;
@@ -191,9 +214,9 @@ FP_HD void phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r
   const int lb = la + 1, lc = la + S::SW, ld = lc + 1;
   const f3 pA = P3(la), pB = P3(lb), pC = P3(lc), pD = P3(ld);
   const f3 uA = V3(la), uB = V3(lb), uC = V3(lc), uD = V3(ld);
-  const bool rowA = Rq >= 0 && Rq < R, colA = Cq >= 0 && Cq < C;
-  const bool quad = Rq >= 0 && Rq < R - 1 && Cq >= 0 && Cq < C - 1;
-  const bool bend = Rq >= 0 && Rq < R - 2 && Cq >= 0 && Cq < C - 2;
+  const bool rowA = INTERIOR || (Rq >= 0 && Rq < R), colA = INTERIOR || (Cq >= 0 && Cq < C);
+  const bool quad = INTERIOR || (Rq >= 0 && Rq < R - 1 && Cq >= 0 && Cq < C - 1);
+  const bool bend = INTERIOR || (Rq >= 0 && Rq < R - 2 && Cq >= 0 && Cq < C - 2);
   const f3 z{0.0f, 0.0f, 0.0f};
 
   // top: row strut A -> B, or for i == 0 the bend strut (r0-2, C') -> Cc
@@ -201,41 +224,51 @@ FP_HD void phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r
   {
     const bool fr = i == 0;
     const int e0 = fr ? la - S::SW : la, e1 = fr ? lc : lb;
-    const KTab K = fr ? L.bv[0] : L.h[j];
-    const bool ok = fr ? (j >= 1 && r0 >= 2 && Cq >= 0 && Cq < C - 2) : (rowA && Cq >= 0 && Cq < C - 1);
+    const KTab K = L.tab[fr ? S::BV0 : S::H0 + j];
     const f3 f = strut_f(P3(e0), V3(e0), P3(e1), V3(e1), K.k, K.c, K.L);
-    f_top = sel3(ok, f, z);
+    if (INTERIOR) {
+      f_top = f;
+    } else {
+      const bool ok = fr ? (j >= 1 && r0 >= 2 && Cq >= 0 && Cq < C - 2) : (rowA && Cq >= 0 && Cq < C - 1);
+      f_top = sel3(ok, f, z);
+    }
   }
   // left: column strut A -> Cc, or for j == 0 the bend strut (R', c0-2) -> B
   f3 f_left;
   {
     const bool fr = j == 0;
     const int e0 = fr ? la - 1 : la, e1 = fr ? lb : lc;
-    const KTab K = fr ? L.bh[0] : L.v[i];
-    const bool ok = fr ? (i >= 1 && c0 >= 2 && Rq >= 0 && Rq < R - 2) : (colA && Rq >= 0 && Rq < R - 1);
+    const KTab K = L.tab[fr ? S::BH0 : S::V0 + i];
     const f3 f = strut_f(P3(e0), V3(e0), P3(e1), V3(e1), K.k, K.c, K.L);
-    f_left = sel3(ok, f, z);
+    if (INTERIOR) {
+      f_left = f;
+    } else {
+      const bool ok = fr ? (i >= 1 && c0 >= 2 && Rq >= 0 && Rq < R - 2) : (colA && Rq >= 0 && Rq < R - 1);
+      f_left = sel3(ok, f, z);
+    }
   }
   // shear: B -> Cc, rest length from the quad's column and row differences
   f3 f_sh;
   {
-    const float dx = L.h[j].d, dy = L.v[i].d;
-    const float Ls = sqrt_rn(dx * dx + dy * dy);
+    const float dx = L.tab[S::H0 + j].d, dy = L.tab[S::V0 + i].d;
+    const float Ls = sqrt_cr(dx * dx + dy * dy);
     const float inv = rcp_f(Ls);
     const f3 f = strut_f(pB, uB, pC, uC, inv * G.sk, inv * G.sc, Ls);
-    f_sh = sel3(quad, f, z);
+    f_sh = INTERIOR ? f : sel3(quad, f, z);
   }
   // bends from A
   f3 f_bh, f_bv;
   {
     const int e1 = la + 2;
-    const KTab K = L.bh[j + 1];
-    f_bh = sel3(bend, strut_f(pA, uA, P3(e1), V3(e1), K.k, K.c, K.L), z);
+    const KTab K = L.tab[S::BH0 + j + 1];
+    const f3 f = strut_f(pA, uA, P3(e1), V3(e1), K.k, K.c, K.L);
+    f_bh = INTERIOR ? f : sel3(bend, f, z);
   }
   {
     const int e1 = la + 2 * S::SW;
-    const KTab K = L.bv[i + 1];
-    f_bv = sel3(bend, strut_f(pA, uA, P3(e1), V3(e1), K.k, K.c, K.L), z);
+    const KTab K = L.tab[S::BV0 + i + 1];
+    const f3 f = strut_f(pA, uA, P3(e1), V3(e1), K.k, K.c, K.L);
+    f_bv = INTERIOR ? f : sel3(bend, f, z);
   }
 
   // faces (B, Cc, A) and (B, D, Cc): drag + lift, corner weights from two angles each
@@ -265,7 +298,7 @@ FP_HD void phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r
     const float wB0 = corner_w(cmB, dB0), wB1 = corner_w(cmB, fdot(eBD, eBD) - dB0);
     const float wB2 = (1.0f - wB0) - wB1;
     const f3 FB = face_f(crB, rsB, vrB, P.neg_drag, P.neg_lift);
-    if (quad) {
+    if (INTERIOR || quad) {
       sA = f3{wA2 * FA.x, wA2 * FA.y, wA2 * FA.z};
       sB = fmad3(wB0, FB, f3{wA0 * FA.x, wA0 * FA.y, wA0 * FA.z});
       sC = fmad3(wB2, FB, f3{wA1 * FA.x, wA1 * FA.y, wA1 * FA.z});
@@ -293,14 +326,14 @@ FP_HD bool pin_bit(const uint32_t* __restrict__ bits, uint32_t i) {
 // Phase 2: thread t = a * TC + b (t < TR * TC) owns tile vertex (r0+a, c0+b): sums the six vectors
 // addressed to it, then gravity, pins, the integrator, collision (`collide(p, v, F, xn, vn)`: the
 // obstacle response of update_points, springy_mesh.rs:423-497) and the store.
-template <int TR, int TC, int MODE, class Collide>
+template <int TR, int TC, int MODE, bool INTERIOR, class Collide>
 FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, const ClothView& G, int r0, int c0,
                   int t, Collide collide) {
   using S = Lds<TR, TC>;
   if (t >= TR * TC) return;
   const int a = t / TC, b = t - a * TC;
   const int r = r0 + a, c = c0 + b;
-  if (r >= (int)G.rows || c >= (int)G.cols) return;
+  if (!INTERIOR && (r >= (int)G.rows || c >= (int)G.cols)) return;
   const uint32_t gi = G.base + (uint32_t)(r * (int)G.cols + c);
   auto get = [&](int o, int q) {
     FP_LDS_CHECK(q >= 0 && q < S::NQ);

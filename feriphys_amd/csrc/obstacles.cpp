@@ -108,6 +108,15 @@ int upload_obstacles(ObstacleSet& s, const std::vector<fp::HostObstacle>& obstac
   O.faces = s.faces.p;
   O.n_faces = (uint32_t)faces.size();
   O.margin = kMargin;
+  for (int k = 0; k < 3; ++k) {  // no obstacles: an empty box
+    s.hit_box.lo[k] = INFINITY;
+    s.hit_box.hi[k] = -INFINITY;
+  }
+  if (faces.empty()) return FP_OK;
+  for (int k = 0; k < 3; ++k) {  // until shown otherwise: everything
+    s.hit_box.lo[k] = -INFINITY;
+    s.hit_box.hi[k] = INFINITY;
+  }
   if (faces.size() <= 8) return FP_OK;  // scan every face in order
 
   std::vector<uint32_t> always, ids[4];
@@ -129,6 +138,25 @@ int upload_obstacles(ObstacleSet& s, const std::vector<fp::HostObstacle>& obstac
   O.n_always = (uint32_t)always.size();
   for (int g = 0; g < 4; ++g)
     if (int e = build_grid(s, g, faces, ids[g], g == 0 ? -1 : g - 1)) return e;
+  if (always.empty()) {
+    // the union of the binned faces' boxes, grown twice by the margin (once as the grids grow them,
+    // once more against the rounding of the grids' cell mapping); a thin face's dropped axis unbounded
+    for (int k = 0; k < 3; ++k) {
+      s.hit_box.lo[k] = INFINITY;
+      s.hit_box.hi[k] = -INFINITY;
+    }
+    for (int g = 0; g < 4; ++g)
+      for (uint32_t f : ids[g]) {
+        const fp::ObsFace& q = faces[f];
+        for (int k = 0; k < 3; ++k) {
+          const bool open = g > 0 && k == g - 1;
+          const float l = open ? -INFINITY : std::min(q.v0[k], std::min(q.v1[k], q.v2[k])) - 2.0f * kMargin;
+          const float h = open ? INFINITY : std::max(q.v0[k], std::max(q.v1[k], q.v2[k])) + 2.0f * kMargin;
+          s.hit_box.lo[k] = std::min(s.hit_box.lo[k], l);
+          s.hit_box.hi[k] = std::max(s.hit_box.hi[k], h);
+        }
+      }
+  }
   return FP_OK;
 }
 

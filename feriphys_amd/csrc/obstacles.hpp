@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "grid_fast.hpp"
 #include "host_util.hpp"
 #include "mesh_host.hpp"
 #include "step_device.hpp"
@@ -16,6 +17,7 @@ struct ObstacleSet {
   DBuf<uint32_t> cell_start[4], cell_faces[4], always;
   uint64_t n_always = 0, n_binned[4] = {0, 0, 0, 0};
   fp::ObsView view{};
+  fp::HitBox hit_box{};  // for the fast kernel's early-out (see fp::HitBox)
   uint64_t n_vertices = 0, n_faces = 0;
 };
 

@@ -303,7 +303,7 @@ int check_sim_mesh(const fp_sim* s, uint64_t mesh) {
 // math form the fast kernel (which has no 16x16 shape: its tiles are built as 15x15).
 hipError_t tiled_launch(fp_sim* s, const fp::StepArgs& a, const fp::TileRange* tiles, uint32_t n, int shape, int mode) {
   if (s->math == FP_MATH_FAST && shape != fp::TILE_16x16)
-    return fp::launch_grid_fast(a, s->cloths.p, tiles, n, shape, mode, s->stream);
+    return fp::launch_grid_fast(a, s->cloths.p, tiles, n, shape, mode, s->obst.hit_box, s->stream);
   return fp::launch_grid_tiled(a, s->cloths.p, tiles, n, shape, mode, s->math == FP_MATH_EXACT, s->stream);
 }
 

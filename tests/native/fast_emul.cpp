@@ -20,19 +20,27 @@
 
 namespace {
 
+template <int TR, int TC, int MODE, bool INTERIOR>
+void tile(fp::gfast::Lds<TR, TC>& L, const fp::StepArgs& A, const fp::ClothView& G, const fp::StepParams& P, int r0,
+          int c0) {
+  const int nthreads = (int)fp::kBlock;
+  for (int t = 0; t < nthreads; ++t) fp::gfast::phase0<TR, TC, INTERIOR>(L, A, G, r0, c0, t, nthreads);
+  for (int t = 0; t < nthreads; ++t) fp::gfast::phase1<TR, TC, INTERIOR>(L, P, G, r0, c0, t);
+  for (int t = 0; t < nthreads; ++t)
+    fp::gfast::phase2<TR, TC, MODE, INTERIOR>(L, A, P, G, r0, c0, t, [](fp::f3, fp::f3, fp::f3, fp::f3&, fp::f3&) {});
+}
+
 template <int TR, int TC, int MODE>
 void run(const fp::StepArgs& A, const fp::ClothView& G, const fp::StepParams& P) {
   using L_t = fp::gfast::Lds<TR, TC>;
   std::unique_ptr<L_t> L(new L_t);
-  const int nthreads = (int)fp::kBlock;
   for (uint32_t r0 = 0; r0 < G.rows; r0 += TR)
     for (uint32_t c0 = 0; c0 < G.cols; c0 += TC) {
       std::memset((void*)L.get(), 0xff, sizeof(L_t));  // NaN everywhere
-      for (int t = 0; t < nthreads; ++t) fp::gfast::phase0<TR, TC>(*L, A, G, (int)r0, (int)c0, t, nthreads);
-      for (int t = 0; t < nthreads; ++t) fp::gfast::phase1<TR, TC>(*L, P, G, (int)r0, (int)c0, t);
-      for (int t = 0; t < nthreads; ++t)
-        fp::gfast::phase2<TR, TC, MODE>(*L, A, P, G, (int)r0, (int)c0, t,
-                                         [](fp::f3, fp::f3, fp::f3, fp::f3&, fp::f3&) {});
+      if (fp::gfast::interior_tile<TR, TC>(G, (int)r0, (int)c0))
+        tile<TR, TC, MODE, true>(*L, A, G, P, (int)r0, (int)c0);
+      else
+        tile<TR, TC, MODE, false>(*L, A, G, P, (int)r0, (int)c0);
     }
 }
 

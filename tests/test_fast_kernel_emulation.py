@@ -3,7 +3,8 @@ CPU against the oracle, before any GPU: every tile's three phases in barrier ord
 index bounds-checked and the LDS filled with NaN (tests/native/fast_emul.cpp).  The same bars as
 the GPU parity tests of the FAST form: one-step forces within 1e-5 of the largest force component,
 one-step states within 1e-5 relative; grids that do not fill their last tiles, both tile shapes,
-pins at the corners and along an edge, non-zero starting forces, Euler and RK4.  (Obstacles are left
+pins at the corners and along an edge, tiles inside the grid (the interior instantiation) and at its
+border, non-zero starting forces, Euler and RK4.  (Obstacles are left
 to the GPU tests: the collision functions are the exact kernels' own.)"""
 import ctypes as C
 import os
@@ -85,7 +86,7 @@ def force_rel_err(f, ref):
 
 CASES = [(32, 32, "corners", 30), (32, 32, "edge", 60), (5, 3, "edge", 10), (2, 2, "edge", 5),
          (19, 129, "edge", 20), (37, 70, "corners", 15), (3, 40, "edge", 10), (16, 16, "edge", 25),
-         (31, 31, "corners", 25), (30, 62, "edge", 12)]
+         (31, 31, "corners", 25), (30, 62, "edge", 12), (64, 96, "edge", 40)]
 
 
 @pytest.mark.parametrize("shape", [0, 1])
