@@ -28,8 +28,8 @@ class SurrogateConfig:
     strain_scale: float = 10.0
     out_scale: float = 0.0  # 0: the library default for dt and ratio
     base_struts: bool = False
-    strain_limit: float = 0.1
-    energy_limit: float = 0.01
+    strain_limit: float = 0.02
+    energy_limit: float = 0.002
 
     def to_c(self) -> _lib.FpSurrogateConfig:
         d = _lib.FpSurrogateConfig()

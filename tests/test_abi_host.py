@@ -257,7 +257,9 @@ def test_surrogate_config_default():
     H = np.float32(np.float32(1.0 / 600.0) * np.float32(8.0))
     assert c.dt == np.float32(1.0 / 600.0) and c.ratio == 8 and c.strain_scale == 10.0
     assert c.out_scale == pytest.approx(float(np.float32(0.005) / (H * H)), rel=1e-6)
-    assert c.base_struts == 0 and c.strain_limit == np.float32(0.1) and c.energy_limit == np.float32(0.01)
+    assert c.base_struts == 0 and c.strain_limit == np.float32(0.02) and c.energy_limit == np.float32(0.002)
+    d = fa.SurrogateConfig()
+    assert (d.strain_limit, d.energy_limit) == (0.02, 0.002)
     assert fa.SurrogateConfig().to_c().out_scale == pytest.approx(c.out_scale, rel=1e-6)
 
 
