@@ -66,6 +66,7 @@ SIGNATURES = {
     "fp_version": (C.c_char_p, []),
     "fp_selftest_exact_math": (C.c_int, [C.POINTER(C.c_uint64)]),
     "fp_sim_count_nonfinite": (C.c_int, [_P, _U64P]),
+    "fp_sim_count_nonfinite_noalloc": (C.c_int, [_P, _U64P]),
     "fp_rigid_config_default": (None, [C.POINTER(FpRigidConfig)]),
     "fp_rigid_new": (C.c_int, [_FP, C.c_float, _PP]),
     "fp_rigid_get_state": (C.c_int, [_P, _FP]),

@@ -828,6 +828,7 @@ int fp_sim_new(fp_mesh* const* meshes, uint64_t n_meshes, fp_obstacle* const* ob
 
   std::unique_ptr<fp_sim> s(new (std::nothrow) fp_sim());
   if (!s) return fail(FP_ENOMEM, "out of host memory");
+  HIP_TRY(hipGetDevice(&s->device));
   HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   HIP_TRY(hipEventCreate(&s->ev0));
   HIP_TRY(hipEventCreate(&s->ev1));

@@ -55,6 +55,7 @@ struct fp_sim {
   std::vector<fpsim::MeshSlot> meshes;
   uint64_t N = 0, F = 0;
   hipStream_t stream = nullptr;
+  int device = 0;  // the device current when the simulation was built (its buffers and stream live there)
   int cur = 0;
   fph::DBuf<float> state[2][6];
   fph::DBuf<float> force[3];

@@ -352,6 +352,13 @@ class Simulation(_Handle):
         check(lib().fp_sim_count_nonfinite(self.handle, _uptr(out)))
         return out[:self.n_meshes]
 
+    def count_nonfinite_noalloc(self) -> np.ndarray:
+        """count_nonfinite through the entry point that allocates no device memory (its counters are
+        a fixed array of the library; at most 65,536 meshes)."""
+        out = np.zeros(max(1, self.n_meshes), np.uint64)
+        check(lib().fp_sim_count_nonfinite_noalloc(self.handle, _uptr(out)))
+        return out[:self.n_meshes]
+
     def time_steps(self, n_steps: int) -> float:
         """Runs n_steps bracketed by HIP events on the simulation's stream; returns milliseconds."""
         ms = C.c_float()

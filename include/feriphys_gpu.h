@@ -177,6 +177,12 @@ int fp_sim_mesh_counts(const fp_sim* sim, uint64_t mesh, uint64_t* n_points, uin
  * per mesh, the number of vertices whose position or velocity holds a NaN or an infinity;
  * counts has one entry per mesh. */
 int fp_sim_count_nonfinite(fp_sim* sim, uint64_t* counts);
+/* No counterpart in the reference.  fp_sim_count_nonfinite without any device allocation, for
+ * callers that cannot call hipMalloc (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation): the per-mesh counters live in a fixed device array of the library, one
+ * per GPU, of at most 65,536 meshes (a larger simulation gets FP_EINVAL).  Calls that overlap on
+ * one GPU share that array and race. */
+int fp_sim_count_nonfinite_noalloc(fp_sim* sim, uint64_t* counts);
 /* get_meshes()[mesh].get_points() (springy/simulation.rs:51-53, springy_mesh.rs:419-421). */
 int fp_sim_get_points(fp_sim* sim, uint64_t mesh, float* records);
 /* Checkpoint restore of one mesh's records (same layout; n must equal its point count). */
