@@ -42,21 +42,28 @@ COMMON = ["step_device.hpp", "vec.hpp", "exact_ops.hpp", "fdlibm_atan2f.hpp", "k
 # while these files still hash to the value it was measured on
 KERNEL_SOURCES = {
     "grid_tiled_kernel<ExactMath,16,16>": ["grid_tiled.hip"] + COMMON,
+    "grid_fast_kernel<15,15>": ["grid_fast.hip", "grid_fast_core.hpp", "grid_fast.hpp", "obstacles.cpp"] + COMMON,
     "patch_kernel<ExactMath>": ["kernels.hip"] + COMMON,
 }
-# rocprofv3 readings per launch: FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the same
-# counter reads for a 256 MiB device copy (2.000 and 1.000), and the VALU issue of the kernel
-# (SQ_INSTS_VALU per wave, GRBM_GUI_ACTIVE per XCD)
+# rocprofv3 readings per dispatch on the 1024 x 1024 cloth (first steps) and the 64 icospheres:
+# FETCH_SIZE and WRITE_SIZE in KiB, each scaled by what the same counter reads for the calibration
+# copy (bench.py --only copy: 2.000 and 1.000); SQ_INSTS_VALU and SQ_WAVES summed over the chip and
+# GRBM_GUI_ACTIVE summed over the 8 XCDs (the commands are in the profiles named)
 MEASURED = {
     ("grid_tiled_kernel<ExactMath,16,16>", "cloth1024"): {
-        "src_sha": "8e8d8a1c89687b1b", "fetch_kib": 13249, "write_kib": 24576,
-        "valu_insts_per_wave": 2177, "waves_per_simd": 16, "active_cycles": 140839,
+        "src_sha": "8e8d8a1c89687b1b", "fetch_kib": 13250, "write_kib": 24576,
+        "valu_insts": 3.5666e7, "waves": 16384, "grbm_gui_active": 1.1305e6,
         "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
+    ("grid_fast_kernel<15,15>", "cloth1024"): {
+        "src_sha": "cbc8ecb8449a786b", "fetch_kib": 13199, "write_kib": 24596,
+        "valu_insts": 1.3601e7, "waves": 19044, "grbm_gui_active": 6.8896e5,
+        "profiles": ["profiles/grid_fast_cloth1024.md"]},
     ("patch_kernel<ExactMath>", "icospheres64"): {
-        "src_sha": "581cd32381980017", "fetch_kib": 216682, "write_kib": 50318,
+        "src_sha": "581cd32381980017", "fetch_kib": 216760, "write_kib": 50327,
         "profiles": ["profiles/batched_and_surfaces.md"]},
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+SIMDS, XCDS = 1024, 8
 
 
 def src_sha(kernel):
@@ -81,19 +88,18 @@ def traffic_of(kernel, workload):
 
 
 def valu_of(kernel, workload):
+    """VALU issue of a kernel: a wave64 f32 instruction holds a 16-lane SIMD for 4 cycles."""
     m = measured(kernel, workload)
-    if m is None or "valu_insts_per_wave" not in m:
+    if m is None or "valu_insts" not in m:
         return None
-    issue = m["valu_insts_per_wave"] * 4 * m["waves_per_simd"]
-    return {"insts_per_wave": m["valu_insts_per_wave"], "issue_cycles_per_simd": issue,
-            "active_cycles": m["active_cycles"], "issue_frac": issue / m["active_cycles"], "profiles": m["profiles"]}
+    issue = m["valu_insts"] * 4 / SIMDS
+    active = m["grbm_gui_active"] / XCDS
+    return {"insts_per_wave": m["valu_insts"] / m["waves"], "issue_cycles_per_simd": issue,
+            "active_cycles": active, "issue_frac": issue / active, "profiles": m["profiles"]}
 
 
-def graph_chunk(steps, chunk):
-    """Steps per captured graph: at most `chunk` and at most `steps`, even (so one graph serves
-    both state-buffer parities' starting point), at least 2."""
-    c = max(2, min(chunk, steps))
-    return c - (c % 2)
+def launch_desc(chunk):
+    return f"{chunk}-step HIP graphs" if chunk > 0 else "eager launches"
 
 
 def time_plain(sim, initial_per_mesh, steps, warmup, chunk):
@@ -105,12 +111,15 @@ def time_plain(sim, initial_per_mesh, steps, warmup, chunk):
         for k, rec in enumerate(initial_per_mesh):
             sim.set_points(k, rec)
     restore()
-    w = max(warmup, chunk)
-    sim.step_graph(-(-w // chunk) * chunk, chunk)
+    if chunk > 0:
+        w = max(warmup, chunk)
+        sim.step_graph(-(-w // chunk) * chunk, chunk)
+    elif warmup:
+        sim.step(warmup)
     restore()
     sim.synchronize()
     t0 = time.perf_counter()
-    kernel_ms = sim.time_steps_graph(steps, chunk)
+    kernel_ms = sim.time_steps_graph(steps, chunk) if chunk > 0 else sim.time_steps(steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
     return wall, kernel_ms
@@ -133,7 +142,10 @@ def run_from(sim, fa, kern, math, initial, steps, chunk):
     import numpy as np
     sim.set_kernel(kern, fa.FP_MATH_EXACT if math == "exact" else fa.FP_MATH_FAST)
     sim.set_points(0, initial)
-    sim.step_graph(steps, chunk)
+    if chunk > 0:
+        sim.step_graph(steps, chunk)
+    else:
+        sim.step(steps)
     return sim.get_points(0)[:, 1:4].astype(np.float64)
 
 
@@ -302,8 +314,7 @@ def headline_line(args, legs, best, sur, compare, extra, N, dt, scene):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"plain springy step, {args.n}x{args.n} grid cloth ({N} vertices), top edge pinned, "
-                               f"320-face icosphere collider, RK4 dt=1/600 s, {graph_chunk(args.steps, args.chunk)}"
-                               f"-step HIP graphs",
+                               f"320-face icosphere collider, RK4 dt=1/600 s, {launch_desc(graph_chunk(args.steps, args.chunk))}",
                    "vertices": N, "dt": dt, "kernel": args.kernel, "math": best,
                    "math_bars": {"exact": "bit-identical to the reference's f32 step",
                                  "fast": "one-step forces and states within 1e-5 relative of the reference"}},
@@ -442,7 +453,7 @@ def batched_plain(args, np, fa, wl):
     chunk = graph_chunk(args.steps, args.chunk)
     rates = plain_rates(sim, fa, initial, args.steps, args.warmup, chunk, N, scene.config.dt, BYTES_PER_VERTEX_STEP * N)
     return {"workload": f"{len(scene.cloths)} grid cloths, sides log-uniform in [8, 256] ({N} vertices), per-cloth "
-                        f"stiffness/pins/wind, RK4 dt=1/600 s, {chunk}-step HIP graphs",
+                        f"stiffness/pins/wind, RK4 dt=1/600 s, {launch_desc(chunk)}",
             "vertices": N, "tile_shape_per_size_class": chosen, "steps": args.steps,
             "bytes_per_vertex_step": BYTES_PER_VERTEX_STEP, "plain": rates}
 
@@ -467,7 +478,7 @@ def surfaces_plain(args, np, fa, wl):
     rates = plain_rates(sim, fa, initial, args.steps, args.warmup, chunk, N, scene.config.dt, bstep)
     rates["exact"]["traffic"] = traffic_of("patch_kernel<ExactMath>", "icospheres64")
     return {"workload": f"64 radially perturbed icospheres (levels 4-7), {N} vertices, {S} struts, {F} faces, ground "
-                        f"plane, RK4 dt=1/600 s, {chunk}-step HIP graphs",
+                        f"plane, RK4 dt=1/600 s, {launch_desc(chunk)}",
             "vertices": N, "struts": S, "faces": F, "steps": args.steps, "bytes_per_step": bstep, "plain": rates}
 
 
@@ -553,7 +564,7 @@ def bench_batched(args, np, fa, wl):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain springy step, {len(scene.cloths)} grid cloths, sides log-uniform in [8, 256] "
                                f"({N} vertices), per-cloth stiffness/pins/wind, RK4 dt=1/600 s, "
-                               f"{chunk}-step HIP graphs", "vertices": N, "math": math,
+                               f"{launch_desc(chunk)}", "vertices": N, "math": math,
                    "tile_shape_per_size_class": chosen},
         "vertex_steps_per_s": args.steps * N / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -632,7 +643,7 @@ def bench_surfaces(args, np, fa, wl):
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"plain springy step, 64 radially perturbed icospheres (levels 4-7), {N} vertices, "
-                               f"{S} struts, {F} faces, ground plane, RK4 dt=1/600 s, {chunk}-step HIP graphs",
+                               f"{S} struts, {F} faces, ground plane, RK4 dt=1/600 s, {launch_desc(chunk)}",
                    "math": math},
         "vertex_steps_per_s": args.steps * N / wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
