@@ -98,6 +98,15 @@ def valu_of(kernel, workload):
             "active_cycles": active, "issue_frac": issue / active, "profiles": m["profiles"]}
 
 
+def graph_chunk(steps, chunk):
+    """Steps per captured graph: at most `chunk` and at most `steps`, even (so one graph serves
+    both state-buffer parities' starting point), at least 2; 0 (eager launches) when `chunk` is 0."""
+    if chunk <= 0:
+        return 0
+    c = max(2, min(chunk, steps))
+    return c - (c % 2)
+
+
 def launch_desc(chunk):
     return f"{chunk}-step HIP graphs" if chunk > 0 else "eager launches"
 

@@ -196,18 +196,29 @@ void cloth_tiles(uint32_t mesh, const fp::ClothDesc& d, int shape, std::vector<f
     for (uint32_t c = 0; c < d.cols; c += tc) out.push_back(fp::TileRange{mesh, r, c, (uint32_t)shape});
 }
 
+// Runs of the tile list per XCD (see xcd_order).
+constexpr size_t kTilePieces = 8;
+
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8), each XCD with its
-// own L2.  Launch order is permuted so that XCD x receives the x-th contiguous eighth of the work
-// list (the first n % 8 eighths one item longer): vertically adjacent tiles, which read each
-// other's halo rows, and neighbouring vertex blocks of a CSR mesh then share an L2.
+// own L2.  Launch order is permuted so that each XCD receives `pieces` contiguous runs of the work
+// list: the list is cut into 8 * pieces runs, XCD x taking runs x, x + 8, x + 16, ... (its total
+// q or q + 1 items for n = 8q + r, the first r XCDs one more).  Within a run, vertically adjacent
+// tiles read each other's halo rows from one L2; several runs per XCD spread work that is not
+// uniform over the list (the tiles over an obstacle) across all eight.
 template <class T>
-std::vector<T> xcd_order(const std::vector<T>& t) {
+std::vector<T> xcd_order(const std::vector<T>& t, size_t pieces = 1) {
   const size_t n = t.size(), q = n / 8, r = n % 8;
+  std::vector<std::vector<T>> per(8);
+  size_t at = 0;
+  for (size_t j = 0; j < pieces; ++j)
+    for (size_t x = 0; x < 8; ++x) {
+      const size_t quota = q + (x < r ? 1 : 0);
+      const size_t len = (j + 1) * quota / pieces - j * quota / pieces;
+      per[x].insert(per[x].end(), t.begin() + at, t.begin() + at + len);
+      at += len;
+    }
   std::vector<T> out(n);
-  for (size_t b = 0; b < n; ++b) {
-    const size_t x = b % 8, k = b / 8;
-    out[b] = t[x * q + (x < r ? x : r) + k];  // k < q + (x < r) for every b < n
-  }
+  for (size_t b = 0; b < n; ++b) out[b] = per[b % 8][b / 8];  // b / 8 < q + (b % 8 < r)
   return out;
 }
 
@@ -277,7 +288,7 @@ int build_block_lists(fp_sim* s) {
   HIP_TRY(s->grid_blocks.upload(g.data(), g.size()));
   HIP_TRY(s->csr_blocks.upload(c.data(), c.size()));
   for (int sh = 0; sh < fp::N_TILE_SHAPES; ++sh) {
-    std::vector<fp::TileRange> o = xcd_order(by_shape[sh]);
+    std::vector<fp::TileRange> o = xcd_order(by_shape[sh], kTilePieces);
     s->n_tiles[sh] = (uint32_t)o.size();
     HIP_TRY(s->tiles[sh].upload(o.data(), o.size()));
   }
