@@ -2,7 +2,7 @@
 // and the launchers of surrogate_kernels.hip.
 //
 // The network maps 72 features of a vertex (its up-to-12 strut neighbours, 6 values each) through
-// three hidden layers of 128 GELU units (a polynomial GELU, see surrogate_kernels.hip) to a
+// three hidden layers of 128 GELU units (a piecewise-quadratic GELU, see surrogate_kernels.hip) to a
 // 3-vector, the correction acceleration for a step of H = ratio * dt.  Weights and activations
 // are bf16, products accumulate in f32 (MFMA).
 #pragma once

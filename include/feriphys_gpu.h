@@ -265,8 +265,9 @@ void fp_sim_free(fp_sim* sim);
  * Learned large-step surrogate.  No counterpart in the reference: the reference only steps with
  * the explicit integrators of state.rs.  A per-vertex network (72 features -> 3 x 128 GELU -> 3;
  * bf16 weights and activations, f32 accumulation on the matrix cores) stands in for the strut
- * forces over a step of H = ratio * dt.  Its GELU is x * clamp(0.5 + x P(x^2), 0, 1) with the
- * quartic P of surrogate_kernels.hip (within 1.3e-3 of x Phi(x)).  The step:
+ * forces over a step of H = ratio * dt.  Its GELU is x * clamp(0.5 + x (a - b min(|x|, x*)), 0, 1),
+ * a C1 piecewise quadratic for the normal CDF (surrogate_kernels.hip; within 0.0197 of x Phi(x)).
+ * The step:
  *     a  = (gravity + face forces [+ strut forces when base_struts]) / m + out_scale * net(f)
  *     v' = v + H a,   x' = x + H v'
  * then the reference's obstacle collision response over H and pinned records kept.  Features of

@@ -1,6 +1,6 @@
 """Pins the surrogate network's CPU oracle (oracle/mlp_oracle.py) before it judges the GPU path:
-the analytic gradient against central finite differences, the polynomial activation against the
-exact (erf) GELU it follows, hand-computed features, and the bf16 rounding rule."""
+the analytic gradient against central finite differences, the piecewise-quadratic activation against
+the exact (erf) GELU it follows, hand-computed features, and the bf16 rounding rule."""
 import numpy as np
 import pytest
 from scipy.special import erf
@@ -18,17 +18,24 @@ def test_bf16_rounding_ties_to_even():
 def test_gelu_tracks_exact_gelu_and_its_derivative():
     z = np.linspace(-8, 8, 16001)
     exact = 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
-    assert np.max(np.abs(mo.gelu(z) - exact)) < 1.4e-3
+    assert np.max(np.abs(mo.gelu(z) - exact)) < 0.0199
     assert mo.gelu(0.0) == 0.0
-    big = np.array([3.5, 5.0, 20.0])
+    big = np.array([2.5, 3.5, 5.0, 20.0])  # beyond z* = 2.463
     np.testing.assert_array_equal(mo.gelu(big), big)  # saturated: exactly x and 0
     np.testing.assert_array_equal(mo.gelu(-big), -0.0 * big)
-    # shape of the exact GELU: minimum near -0.75 of about -0.17, monotone above it
-    assert abs(z[np.argmin(mo.gelu(z))] + 0.75) < 0.03 and abs(mo.gelu(z).min() + 0.17) < 2e-3
+    # on (-z*, 0) act = z/2 + a z^2 + b z^3: its minimum is the root of 1/2 + 2 a z + 3 b z^2
+    a, b = mo.A, mo.BX * mo.IXS
+    zmin = (-2 * a + np.sqrt(4 * a * a - 6 * b)) / (6 * b)
+    assert abs(z[np.argmin(mo.gelu(z))] - zmin) < 1e-3 and abs(zmin + 0.821) < 1e-3
+    assert abs(mo.gelu(zmin) - (zmin / 2 + a * zmin ** 2 + b * zmin ** 3)) < 1e-12
+    assert abs(mo.gelu(zmin) + 0.17) < 0.013  # the exact GELU's minimum is -0.170 at -0.752
     h = 1e-6
-    zz = z[np.abs(np.abs(z) - 3.361) > 1e-3]  # away from the two clamp points
+    zz = z[np.abs(np.abs(z) - mo.XS) > 1e-3]  # away from the two clamp points
     fd = (mo.gelu(zz + h) - mo.gelu(zz - h)) / (2 * h)
     np.testing.assert_allclose(mo.gelu_grad(zz), fd, rtol=0, atol=1e-7)
+    # C1: the slope is continuous through the clamp points (1 at z*, 0 at -z*)
+    for zc, want in ((mo.XS, 1.0), (-mo.XS, 0.0)):
+        np.testing.assert_allclose(mo.gelu_grad(np.array([zc - 1e-5, zc + 1e-5])), [want, want], atol=1e-4)
 
 
 def test_features_hand_case():
