@@ -31,15 +31,15 @@ __global__ void __launch_bounds__(kBlock) grid_fast_kernel(StepArgs A, const Clo
   if (gfast::interior_tile<TR, TC>(G, r0, c0)) {
     gfast::phase0<TR, TC, true>(L, A, G, r0, c0, tid, (int)kBlock);
     __syncthreads();
-    gfast::phase1<TR, TC, true>(L, P, G, r0, c0, tid);
+    const f3 toA = gfast::phase1<TR, TC, true>(L, P, G, r0, c0, tid);
     __syncthreads();
-    gfast::phase2<TR, TC, MODE, true>(L, A, P, G, r0, c0, tid, collide);
+    gfast::phase2<TR, TC, MODE, true>(L, A, P, G, r0, c0, tid, toA, collide);
   } else {
     gfast::phase0<TR, TC, false>(L, A, G, r0, c0, tid, (int)kBlock);
     __syncthreads();
-    gfast::phase1<TR, TC, false>(L, P, G, r0, c0, tid);
+    const f3 toA = gfast::phase1<TR, TC, false>(L, P, G, r0, c0, tid);
     __syncthreads();
-    gfast::phase2<TR, TC, MODE, false>(L, A, P, G, r0, c0, tid, collide);
+    gfast::phase2<TR, TC, MODE, false>(L, A, P, G, r0, c0, tid, toA, collide);
   }
 }
 

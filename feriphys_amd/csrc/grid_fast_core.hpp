@@ -145,7 +145,7 @@ struct Lds {
   float4 pv[SH * SW];  // x, y, z, vx
   float2 vw[SH * SW];  // vy, vz
   KTab tab[NTAB];
-  float out[6][3][NQ];  // per quad thread: to_A, to_B, to_C, to_D, bend far ends (row, column)
+  float out[5][3][NQ];  // per quad thread: to_B, to_C, to_D, bend far ends (row, column); to_A stays in registers
 };
 
 // Whether a tile's staged region (the tile and a 2-vertex border) lies inside its grid.
@@ -170,14 +170,30 @@ template <int TR, int TC, bool INTERIOR>
 FP_HD void phase0(Lds<TR, TC>& L, const StepArgs& A, const ClothView& G, int r0, int c0, int tid, int nthreads) {
   using S = Lds<TR, TC>;
   const int R = (int)G.rows, C = (int)G.cols;
-  for (int e = tid; e < S::SH * S::SW; e += nthreads) {
-    const int lr = e / S::SW, lc = e - lr * S::SW;
-    const int r = r0 - 2 + lr, c = c0 - 2 + lc;
-    if (!INTERIOR && (r < 0 || r >= R || c < 0 || c >= C)) continue;
-    const uint32_t g = G.base + (uint32_t)(r * C + c);
-    FP_LDS_CHECK(e >= 0 && e < S::SH * S::SW);
-    L.pv[e] = float4{A.in.x[g], A.in.y[g], A.in.z[g], A.in.vx[g]};
-    L.vw[e] = float2{A.in.vy[g], A.in.vz[g]};
+  if (INTERIOR) {
+    // the staged region's first vertex (r0-2, c0-2) lies in the grid: every array is addressed
+    // from its (workgroup-uniform) pointer to that vertex plus a 32-bit byte offset per lane
+    const uint32_t g0 = G.base + (uint32_t)((r0 - 2) * C + (c0 - 2));
+    const char *bx = (const char*)(A.in.x + g0), *by = (const char*)(A.in.y + g0), *bz = (const char*)(A.in.z + g0);
+    const char *bu = (const char*)(A.in.vx + g0), *bv = (const char*)(A.in.vy + g0), *bw = (const char*)(A.in.vz + g0);
+    for (int e = tid; e < S::SH * S::SW; e += nthreads) {
+      const int lr = e / S::SW, lc = e - lr * S::SW;
+      const uint32_t ob = (uint32_t)(lr * C + lc) * 4u;
+      FP_LDS_CHECK(e >= 0 && e < S::SH * S::SW);
+      L.pv[e] = float4{*(const float*)(bx + ob), *(const float*)(by + ob), *(const float*)(bz + ob),
+                       *(const float*)(bu + ob)};
+      L.vw[e] = float2{*(const float*)(bv + ob), *(const float*)(bw + ob)};
+    }
+  } else {
+    for (int e = tid; e < S::SH * S::SW; e += nthreads) {
+      const int lr = e / S::SW, lc = e - lr * S::SW;
+      const int r = r0 - 2 + lr, c = c0 - 2 + lc;
+      if (r < 0 || r >= R || c < 0 || c >= C) continue;
+      const uint32_t g = G.base + (uint32_t)(r * C + c);
+      FP_LDS_CHECK(e >= 0 && e < S::SH * S::SW);
+      L.pv[e] = float4{A.in.x[g], A.in.y[g], A.in.z[g], A.in.vx[g]};
+      L.vw[e] = float2{A.in.vy[g], A.in.vz[g]};
+    }
   }
   for (int e = tid; e < S::NTAB; e += nthreads) {
     // which table: coordinate axis (columns or rows), first index, reach (1 or 2), nominal constants
@@ -191,11 +207,12 @@ FP_HD void phase0(Lds<TR, TC>& L, const StepArgs& A, const ClothView& G, int r0,
   }
 }
 
-// Phase 1: thread t = i * QC + j evaluates quad (r0-1+i, c0-1+j)'s struts and faces.
+// Phase 1: thread t = i * QC + j evaluates quad (r0-1+i, c0-1+j)'s struts and faces; it returns what
+// the quad owes its corner A, the vertex the same thread owns in phase 2 (for i, j >= 1).
 template <int TR, int TC, bool INTERIOR>
-FP_HD void phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r0, int c0, int t) {
+FP_HD f3 phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r0, int c0, int t) {
   using S = Lds<TR, TC>;
-  if (t >= S::NQ) return;
+  if (t >= S::NQ) return f3{0.0f, 0.0f, 0.0f};
   const int R = (int)G.rows, C = (int)G.cols;
   const int i = t / S::QC, j = t - i * S::QC;
   const int Rq = r0 - 1 + i, Cq = c0 - 1 + j;  // corner A
@@ -310,28 +327,32 @@ FP_HD void phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r
   const f3 toA = add(add(add(add(sA, f_top), f_left), f_bh), f_bv);
   const f3 toB = sub(sub(add(sB, f_sh), f_top), sel3(j == 0, f_left, z));
   const f3 toC = sub(sub(sub(sC, f_sh), f_left), sel3(i == 0, f_top, z));
-  const f3 outs[6] = {toA, toB, toC, sD, neg(f_bh), neg(f_bv)};
+  const f3 outs[5] = {toB, toC, sD, f_bh, f_bv};  // the bends' far ends take -f: phase 2 subtracts
 #pragma unroll
-  for (int o = 0; o < 6; ++o) {
+  for (int o = 0; o < 5; ++o) {
     L.out[o][0][t] = outs[o].x;
     L.out[o][1][t] = outs[o].y;
     L.out[o][2][t] = outs[o].z;
   }
+  return toA;
 }
 
 FP_HD bool pin_bit(const uint32_t* __restrict__ bits, uint32_t i) {
   return bits != nullptr && ((bits[i >> 5] >> (i & 31u)) & 1u);
 }
 
-// Phase 2: thread t = a * TC + b (t < TR * TC) owns tile vertex (r0+a, c0+b): sums the six vectors
-// addressed to it, then gravity, pins, the integrator, collision (`collide(p, v, F, xn, vn)`: the
-// obstacle response of update_points, springy_mesh.rs:423-497) and the store.
+// Phase 2: thread t = i * QC + j with i, j >= 1 owns tile vertex (r0+i-1, c0+j-1), corner A of its
+// own phase-1 quad (toA, passed in): it sums the vectors addressed to the vertex, then gravity,
+// pins, the integrator, collision (`collide(p, v, F, xn, vn)`: the obstacle response of
+// update_points, springy_mesh.rs:423-497) and the store.
 template <int TR, int TC, int MODE, bool INTERIOR, class Collide>
 FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, const ClothView& G, int r0, int c0,
-                  int t, Collide collide) {
+                  int t, f3 toA, Collide collide) {
   using S = Lds<TR, TC>;
-  if (t >= TR * TC) return;
-  const int a = t / TC, b = t - a * TC;
+  if (t >= S::NQ) return;
+  const int i = t / S::QC, j = t - i * S::QC;
+  if (i == 0 || j == 0) return;
+  const int a = i - 1, b = j - 1;
   const int r = r0 + a, c = c0 + b;
   if (!INTERIOR && (r >= (int)G.rows || c >= (int)G.cols)) return;
   const uint32_t gi = G.base + (uint32_t)(r * (int)G.cols + c);
@@ -339,16 +360,16 @@ FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, 
     FP_LDS_CHECK(q >= 0 && q < S::NQ);
     return f3{L.out[o][0][q], L.out[o][1][q], L.out[o][2][q]};
   };
-  const int qA = (a + 1) * S::QC + (b + 1), qB = (a + 1) * S::QC + b, qC = a * S::QC + (b + 1), qD = a * S::QC + b;
+  // the quads with this vertex as corner B, Cc and D: threads t - 1, t - QC, t - QC - 1
   f3 F = A.force_init ? f3{A.fx[gi], A.fy[gi], A.fz[gi]} : f3{0.0f, 0.0f, 0.0f};
   F = add(F, P.gravity);
-  F = add(F, get(0, qA));
-  F = add(F, get(1, qB));
-  F = add(F, get(2, qC));
-  F = add(F, get(3, qD));
+  F = add(F, toA);
+  F = add(F, get(0, t - 1));
+  F = add(F, get(1, t - S::QC));
+  F = add(F, get(2, t - S::QC - 1));
   const f3 z{0.0f, 0.0f, 0.0f};
-  F = add(F, b >= 1 ? get(4, qB - 1) : z);          // bend (r, c-2) -> (r, c): thread (a+1, b-1)
-  F = add(F, a >= 1 ? get(5, qC - S::QC) : z);      // bend (r-2, c) -> (r, c): thread (a-1, b+1)
+  F = sub(F, j >= 2 ? get(3, t - 2) : z);           // bend (r, c-2) -> (r, c): thread (i, j-2)
+  F = sub(F, i >= 2 ? get(4, t - 2 * S::QC) : z);   // bend (r-2, c) -> (r, c): thread (i-2, j)
   const bool pin = pin_bit(A.pins, gi);
   if (pin) F = z;
   if (MODE == MODE_FORCES) {
@@ -376,12 +397,22 @@ FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, 
     xn = p;
     vn = v;
   }
-  A.out.x[gi] = xn.x;
-  A.out.y[gi] = xn.y;
-  A.out.z[gi] = xn.z;
-  A.out.vx[gi] = vn.x;
-  A.out.vy[gi] = vn.y;
-  A.out.vz[gi] = vn.z;
+  if (INTERIOR) {  // the tile's first vertex as a workgroup-uniform base, a 32-bit offset per lane
+    const uint32_t g0 = G.base + (uint32_t)(r0 * (int)G.cols + c0), ob = (uint32_t)(a * (int)G.cols + b) * 4u;
+    *(float*)((char*)(A.out.x + g0) + ob) = xn.x;
+    *(float*)((char*)(A.out.y + g0) + ob) = xn.y;
+    *(float*)((char*)(A.out.z + g0) + ob) = xn.z;
+    *(float*)((char*)(A.out.vx + g0) + ob) = vn.x;
+    *(float*)((char*)(A.out.vy + g0) + ob) = vn.y;
+    *(float*)((char*)(A.out.vz + g0) + ob) = vn.z;
+  } else {
+    A.out.x[gi] = xn.x;
+    A.out.y[gi] = xn.y;
+    A.out.z[gi] = xn.z;
+    A.out.vx[gi] = vn.x;
+    A.out.vy[gi] = vn.y;
+    A.out.vz[gi] = vn.z;
+  }
   if (A.force_init) {
     A.fx[gi] = 0.0f;
     A.fy[gi] = 0.0f;

@@ -25,9 +25,11 @@ void tile(fp::gfast::Lds<TR, TC>& L, const fp::StepArgs& A, const fp::ClothView&
           int c0) {
   const int nthreads = (int)fp::kBlock;
   for (int t = 0; t < nthreads; ++t) fp::gfast::phase0<TR, TC, INTERIOR>(L, A, G, r0, c0, t, nthreads);
-  for (int t = 0; t < nthreads; ++t) fp::gfast::phase1<TR, TC, INTERIOR>(L, P, G, r0, c0, t);
+  std::vector<fp::f3> toA(nthreads);  // each thread's registers across the barrier
+  for (int t = 0; t < nthreads; ++t) toA[t] = fp::gfast::phase1<TR, TC, INTERIOR>(L, P, G, r0, c0, t);
   for (int t = 0; t < nthreads; ++t)
-    fp::gfast::phase2<TR, TC, MODE, INTERIOR>(L, A, P, G, r0, c0, t, [](fp::f3, fp::f3, fp::f3, fp::f3&, fp::f3&) {});
+    fp::gfast::phase2<TR, TC, MODE, INTERIOR>(L, A, P, G, r0, c0, t, toA[t],
+                                              [](fp::f3, fp::f3, fp::f3, fp::f3&, fp::f3&) {});
 }
 
 template <int TR, int TC, int MODE>
