@@ -84,34 +84,36 @@ __device__ __forceinline__ float unit_open(uint64_t h) { return ((float)(h >> 40
 
 constexpr int kMlpTile = 512;
 
-// acc = W B + bias (the bias is the accumulators' initial value)
+// One layer, acc = W B + bias then the activation, pipelined by output block: the 16 neurons of
+// block mt accumulate (KB MFMAs per column tile, the bias as the accumulators' initial value)
+// while the activation of block mt - 1 is evaluated, so that each wave keeps the matrix cores and
+// the vector ALUs busy together; two accumulator blocks are live instead of eight.  `out` must not
+// alias `B`, which is read until the last block.
 template <int KB, int LDW, int NT>
-__device__ __forceinline__ void mfma_layer(const __bf16* __restrict__ W, const float* __restrict__ bias,
-                                           const bf16x8 (&B)[KB][NT], f32x4 (&acc)[8][NT], int col, int g) {
+__device__ __forceinline__ void mfma_act_layer(const __bf16* __restrict__ W, const float* __restrict__ bias,
+                                               const bf16x8 (&B)[KB][NT], bf16x8 (&out)[4][NT], int col, int g) {
+  f32x4 acc[2][NT];
 #pragma unroll
-  for (int mt = 0; mt < 8; ++mt) {
-    const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
+  for (int mt = 0; mt <= 8; ++mt) {
+    if (mt < 8) {
+      const f32x4 b = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = b;
-  }
+      for (int nt = 0; nt < NT; ++nt) acc[mt & 1][nt] = b;
 #pragma unroll
-  for (int kb = 0; kb < KB; ++kb)
+      for (int kb = 0; kb < KB; ++kb) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * mt + col) * LDW + kb * 32 + g * 8);
 #pragma unroll
-    for (int mt = 0; mt < 8; ++mt) {
-      const bf16x8 a = *reinterpret_cast<const bf16x8*>(W + (16 * mt + col) * LDW + kb * 32 + g * 8);
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = mfma(a, B[kb][nt], acc[mt][nt]);
+        for (int nt = 0; nt < NT; ++nt) acc[mt & 1][nt] = mfma(a, B[kb][nt], acc[mt & 1][nt]);
+      }
     }
-}
-
-template <int NT>
-__device__ __forceinline__ void gelu_pack(const f32x4 (&acc)[8][NT], bf16x8 (&out)[4][NT]) {
+    if (mt > 0) {
+      const int p = mt - 1;
 #pragma unroll
-  for (int mt = 0; mt < 8; ++mt)
+      for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) out[mt >> 1][nt][(mt & 1) * 4 + r] = (__bf16)gelu(acc[mt][nt][r]);
+        for (int r = 0; r < 4; ++r) out[p >> 1][nt][(p & 1) * 4 + r] = (__bf16)gelu(acc[p & 1][nt][r]);
+    }
+  }
 }
 
 template <int NT>
@@ -165,14 +167,10 @@ __global__ void __launch_bounds__(2048 / NT) sur_mlp_kernel(SoA S, const SlotEnt
 #pragma unroll
         for (int s = 0; s < 8; ++s) B1[kb][nt][s] = (__bf16)f[kb * 8 + s];
     }
-    f32x4 acc[8][NT];
-    bf16x8 Bh[4][NT];
-    mfma_layer<3, kLd1, NT>(W1, Bias, B1, acc, col, g);
-    gelu_pack<NT>(acc, Bh);
-    mfma_layer<4, kLd, NT>(W2, Bias + kHid, Bh, acc, col, g);
-    gelu_pack<NT>(acc, Bh);
-    mfma_layer<4, kLd, NT>(W3, Bias + 2 * kHid, Bh, acc, col, g);
-    gelu_pack<NT>(acc, Bh);
+    bf16x8 Bh[4][NT], Bg[4][NT];
+    mfma_act_layer<3, kLd1, NT>(W1, Bias, B1, Bh, col, g);
+    mfma_act_layer<4, kLd, NT>(W2, Bias + kHid, Bh, Bg, col, g);
+    mfma_act_layer<4, kLd, NT>(W3, Bias + 2 * kHid, Bg, Bh, col, g);
     f32x4 o[NT];
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) o[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
