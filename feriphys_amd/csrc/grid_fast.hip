@@ -1,6 +1,6 @@
 // Fast-math LDS-tiled step kernel for Cloth::new meshes: one workgroup per tile, the three phases
-// of grid_fast_core.hpp separated by barriers.  Tiles are dealt to the XCDs in contiguous bands by
-// the host (as for the exact tiled kernel), so vertically adjacent tiles share an L2.
+// of grid_fast_core.hpp separated by barriers.  The host deals the tiles to the XCDs in runs of
+// consecutive tiles (as for the exact tiled kernel), so vertically adjacent tiles mostly share an L2.
 #include "grid_fast.hpp"
 #include "grid_fast_core.hpp"
 
