@@ -195,7 +195,10 @@ FP_HD void phase0(Lds<TR, TC>& L, const StepArgs& A, const ClothView& G, int r0,
       L.vw[e] = float2{A.in.vy[g], A.in.vz[g]};
     }
   }
-  for (int e = tid; e < S::NTAB; e += nthreads) {
+  // the tables are computed by the upper half of the workgroup (waves 2 and 3 of 4), whose share of
+  // the staging above is one round of loads where the lower half has two
+  const int t0 = nthreads >= 2 * S::NTAB ? nthreads / 2 : 0;
+  for (int e = tid - t0; e >= 0 && e < S::NTAB; e += nthreads) {
     // which table: coordinate axis (columns or rows), first index, reach (1 or 2), nominal constants
     const bool bend = e >= S::BH0;
     const int q = e - (e >= S::BV0 ? S::BV0 : e >= S::BH0 ? S::BH0 : e >= S::V0 ? S::V0 : S::H0);
