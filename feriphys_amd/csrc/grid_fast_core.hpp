@@ -292,7 +292,7 @@ FP_HD f3 phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r0,
   }
 
   // faces (B, Cc, A) and (B, D, Cc): drag + lift, corner weights from two angles each
-  f3 sA = z, sB = z, sC = z, sD = z;
+  f3 sA, sB, sC, sD;
   {
     const f3 eBC = fsub(pC, pB), eBA = fsub(pA, pB), eBD = fsub(pD, pB);
     const float third = 1.0f / 3.0f;
@@ -318,11 +318,15 @@ FP_HD f3 phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r0,
     const float wB0 = corner_w(cmB, dB0), wB1 = corner_w(cmB, fdot(eBD, eBD) - dB0);
     const float wB2 = (1.0f - wB0) - wB1;
     const f3 FB = face_f(crB, rsB, vrB, P.neg_drag, P.neg_lift);
-    if (INTERIOR || quad) {
-      sA = f3{wA2 * FA.x, wA2 * FA.y, wA2 * FA.z};
-      sB = fmad3(wB0, FB, f3{wA0 * FA.x, wA0 * FA.y, wA0 * FA.z});
-      sC = fmad3(wB2, FB, f3{wA1 * FA.x, wA1 * FA.y, wA1 * FA.z});
-      sD = f3{wB1 * FB.x, wB1 * FB.y, wB1 * FB.z};
+    sA = f3{wA2 * FA.x, wA2 * FA.y, wA2 * FA.z};
+    sB = fmad3(wB0, FB, f3{wA0 * FA.x, wA0 * FA.y, wA0 * FA.z});
+    sC = fmad3(wB2, FB, f3{wA1 * FA.x, wA1 * FA.y, wA1 * FA.z});
+    sD = f3{wB1 * FB.x, wB1 * FB.y, wB1 * FB.z};
+    if (!INTERIOR) {
+      sA = sel3(quad, sA, z);
+      sB = sel3(quad, sB, z);
+      sC = sel3(quad, sC, z);
+      sD = sel3(quad, sD, z);
     }
   }
 
@@ -359,12 +363,16 @@ FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, 
   const int r = r0 + a, c = c0 + b;
   if (!INTERIOR && (r >= (int)G.rows || c >= (int)G.cols)) return;
   const uint32_t gi = G.base + (uint32_t)(r * (int)G.cols + c);
+  // the vertex's records: the tile's first vertex (r0, c0), which lies in the grid, as a
+  // workgroup-uniform base and a 32-bit byte offset per lane (no per-lane 64-bit addresses)
+  const uint32_t g0 = G.base + (uint32_t)(r0 * (int)G.cols + c0), ob = (uint32_t)(a * (int)G.cols + b) * 4u;
+  auto rec = [&](float* base) -> float& { return *(float*)((char*)(base + g0) + ob); };
   auto get = [&](int o, int q) {
     FP_LDS_CHECK(q >= 0 && q < S::NQ);
     return f3{L.out[o][0][q], L.out[o][1][q], L.out[o][2][q]};
   };
   // the quads with this vertex as corner B, Cc and D: threads t - 1, t - QC, t - QC - 1
-  f3 F = A.force_init ? f3{A.fx[gi], A.fy[gi], A.fz[gi]} : f3{0.0f, 0.0f, 0.0f};
+  f3 F = A.force_init ? f3{rec(A.fx), rec(A.fy), rec(A.fz)} : f3{0.0f, 0.0f, 0.0f};
   F = add(F, P.gravity);
   F = add(F, toA);
   F = add(F, get(0, t - 1));
@@ -376,9 +384,9 @@ FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, 
   const bool pin = pin_bit(A.pins, gi);
   if (pin) F = z;
   if (MODE == MODE_FORCES) {
-    A.fx[gi] = F.x;
-    A.fy[gi] = F.y;
-    A.fz[gi] = F.z;
+    rec(A.fx) = F.x;
+    rec(A.fy) = F.y;
+    rec(A.fz) = F.z;
     return;
   }
   const int ls = (a + 2) * S::SW + (b + 2);
@@ -400,26 +408,16 @@ FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, 
     xn = p;
     vn = v;
   }
-  if (INTERIOR) {  // the tile's first vertex as a workgroup-uniform base, a 32-bit offset per lane
-    const uint32_t g0 = G.base + (uint32_t)(r0 * (int)G.cols + c0), ob = (uint32_t)(a * (int)G.cols + b) * 4u;
-    *(float*)((char*)(A.out.x + g0) + ob) = xn.x;
-    *(float*)((char*)(A.out.y + g0) + ob) = xn.y;
-    *(float*)((char*)(A.out.z + g0) + ob) = xn.z;
-    *(float*)((char*)(A.out.vx + g0) + ob) = vn.x;
-    *(float*)((char*)(A.out.vy + g0) + ob) = vn.y;
-    *(float*)((char*)(A.out.vz + g0) + ob) = vn.z;
-  } else {
-    A.out.x[gi] = xn.x;
-    A.out.y[gi] = xn.y;
-    A.out.z[gi] = xn.z;
-    A.out.vx[gi] = vn.x;
-    A.out.vy[gi] = vn.y;
-    A.out.vz[gi] = vn.z;
-  }
+  rec(A.out.x) = xn.x;
+  rec(A.out.y) = xn.y;
+  rec(A.out.z) = xn.z;
+  rec(A.out.vx) = vn.x;
+  rec(A.out.vy) = vn.y;
+  rec(A.out.vz) = vn.z;
   if (A.force_init) {
-    A.fx[gi] = 0.0f;
-    A.fy[gi] = 0.0f;
-    A.fz[gi] = 0.0f;
+    rec(A.fx) = 0.0f;
+    rec(A.fy) = 0.0f;
+    rec(A.fz) = 0.0f;
   }
 }
 
