@@ -8,10 +8,10 @@ namespace fp {
 
 namespace {
 
-// Six waves per SIMD: 80 VGPRs per lane (the interior-tile path needs no more; the outer ring's
-// bounds-checked path spills a few to scratch) and 6 x 25 KB of LDS, six workgroups per CU.
+// Seven waves per SIMD: at most 72 VGPRs per lane, and with 15x15 tiles 22.7 KiB of LDS per
+// workgroup, seven workgroups per CU (7x31 tiles take 23.0 KiB: six).
 template <int TR, int TC, int MODE>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6, 8)))
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
 grid_fast_kernel(StepArgs A, const ClothView* __restrict__ cloths, const TileRange* __restrict__ tiles, HitBox hb) {
   static_assert((TR + 1) * (TC + 1) == (int)kBlock, "one quad per thread");
   __shared__ gfast::Lds<TR, TC> L;

@@ -139,13 +139,17 @@ struct Lds {
   static constexpr int SH = TR + 4, SW = TC + 4;  // staged rows [r0-2, r0+TR+2), cols [c0-2, c0+TC+2)
   static constexpr int QR = TR + 1, QC = TC + 1;  // quads = threads
   static constexpr int NQ = QR * QC;
+  static constexpr int NV = TR * TC;              // tile vertices
   // strut tables, one array: row struts by first column c0-1+j (at H0), column struts by first row
   // r0-1+i (V0), row bends by first column c0-2+j (BH0), column bends by first row r0-2+i (BV0)
   static constexpr int H0 = 0, V0 = QC, BH0 = QC + QR, BV0 = 2 * QC + QR + 1, NTAB = 2 * (QC + QR) + 2;
   float4 pv[SH * SW];  // x, y, z, vx
   float2 vw[SH * SW];  // vy, vz
   KTab tab[NTAB];
-  float out[5][3][NQ];  // per quad thread: to_B, to_C, to_D, bend far ends (row, column); to_A stays in registers
+  // what the quads owe the tile vertices, by recipient (tile-local vertex (a, b) at a * TC + b):
+  // from the quad with the vertex as corner B, as corner Cc, as corner D, and the far ends of the
+  // row and column bends reaching it (to_A stays in its thread's registers)
+  float out[5][3][NV];
 };
 
 // Whether a tile's staged region (the tile and a 2-vertex border) lies inside its grid.
@@ -334,13 +338,22 @@ FP_HD f3 phase1(Lds<TR, TC>& L, const StepParams& P, const ClothView& G, int r0,
   const f3 toA = add(add(add(add(sA, f_top), f_left), f_bh), f_bv);
   const f3 toB = sub(sub(add(sB, f_sh), f_top), sel3(j == 0, f_left, z));
   const f3 toC = sub(sub(sub(sC, f_sh), f_left), sel3(i == 0, f_top, z));
-  const f3 outs[5] = {toB, toC, sD, f_bh, f_bv};  // the bends' far ends take -f: phase 2 subtracts
-#pragma unroll
-  for (int o = 0; o < 5; ++o) {
-    L.out[o][0][t] = outs[o].x;
-    L.out[o][1][t] = outs[o].y;
-    L.out[o][2][t] = outs[o].z;
-  }
+  // stored by recipient: B = (i-1, j), Cc = (i, j-1), D = (i, j), the row bend's far end (i-1, j+1)
+  // and the column bend's (i+1, j-1) in tile-local coordinates, skipped outside the tile (the bends'
+  // far ends take -f: phase 2 subtracts)
+  auto put = [&](int o, bool in_tile, int slot, f3 f) {
+    if (in_tile) {
+      FP_LDS_CHECK(slot >= 0 && slot < S::NV);
+      L.out[o][0][slot] = f.x;
+      L.out[o][1][slot] = f.y;
+      L.out[o][2][slot] = f.z;
+    }
+  };
+  put(0, i >= 1 && j < TC, (i - 1) * TC + j, toB);
+  put(1, i < TR && j >= 1, i * TC + j - 1, toC);
+  put(2, i < TR && j < TC, i * TC + j, sD);
+  put(3, i >= 1 && j + 1 < TC, (i - 1) * TC + j + 1, f_bh);
+  put(4, i + 1 < TR && j >= 1, (i + 1) * TC + j - 1, f_bv);
   return toA;
 }
 
@@ -367,20 +380,22 @@ FP_HD void phase2(const Lds<TR, TC>& L, const StepArgs& A, const StepParams& P, 
   // workgroup-uniform base and a 32-bit byte offset per lane (no per-lane 64-bit addresses)
   const uint32_t g0 = G.base + (uint32_t)(r0 * (int)G.cols + c0), ob = (uint32_t)(a * (int)G.cols + b) * 4u;
   auto rec = [&](float* base) -> float& { return *(float*)((char*)(base + g0) + ob); };
-  auto get = [&](int o, int q) {
-    FP_LDS_CHECK(q >= 0 && q < S::NQ);
-    return f3{L.out[o][0][q], L.out[o][1][q], L.out[o][2][q]};
+  const int slot = a * TC + b;  // the vertex's slot in L.out
+  auto get = [&](int o) {
+    FP_LDS_CHECK(slot >= 0 && slot < S::NV);
+    return f3{L.out[o][0][slot], L.out[o][1][slot], L.out[o][2][slot]};
   };
-  // the quads with this vertex as corner B, Cc and D: threads t - 1, t - QC, t - QC - 1
+  // then what the quads with this vertex as corner B, Cc and D (threads t - 1, t - QC, t - QC - 1)
+  // and the bends ending at it left in its slot
   f3 F = A.force_init ? f3{rec(A.fx), rec(A.fy), rec(A.fz)} : f3{0.0f, 0.0f, 0.0f};
   F = add(F, P.gravity);
   F = add(F, toA);
-  F = add(F, get(0, t - 1));
-  F = add(F, get(1, t - S::QC));
-  F = add(F, get(2, t - S::QC - 1));
+  F = add(F, get(0));
+  F = add(F, get(1));
+  F = add(F, get(2));
   const f3 z{0.0f, 0.0f, 0.0f};
-  F = sub(F, j >= 2 ? get(3, t - 2) : z);           // bend (r, c-2) -> (r, c): thread (i, j-2)
-  F = sub(F, i >= 2 ? get(4, t - 2 * S::QC) : z);   // bend (r-2, c) -> (r, c): thread (i-2, j)
+  F = sub(F, b >= 1 ? get(3) : z);  // bend (r, c-2) -> (r, c) from thread (i, j-2); for b == 0 a left strut
+  F = sub(F, a >= 1 ? get(4) : z);  // bend (r-2, c) -> (r, c) from thread (i-2, j); for a == 0 a top strut
   const bool pin = pin_bit(A.pins, gi);
   if (pin) F = z;
   if (MODE == MODE_FORCES) {
