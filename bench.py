@@ -17,8 +17,9 @@ Prints ONE JSON line:
                   kernel's average duration measured with HIP events on the simulation's stream
   surrogate       the learned big step over the full 600-step horizon (75 big steps of 8 dt),
                   compared with the exact plain run after the same number of dt
-  batched,        the plain rates of the 4,096-cloth batch and the 64-icosphere workload, with
-  surfaces        their rooflines (omitted with --no-extra)
+  batched,        the plain rates of the 4,096-cloth batch, the 64-icosphere workload and the
+  surfaces,       1,048,576 rigid cubes, with their rooflines (omitted with --no-extra)
+  rigid
   cpu_baseline    the oracle (a single-threaded C++ restatement of the reference step) on a bounded
                   sample of the same workload, on this host's cores
 """
@@ -178,7 +179,7 @@ def main():
                     help="plain: only the timed plain steps of the cloth (for profiling the step kernel); "
                          "mlp: only the surrogate network kernel over the cloth's vertices, --steps launches; "
                          "copy: --steps device copies of 256 MiB (the calibration of the HBM byte counters)")
-    ap.add_argument("--no-extra", action="store_true", help="leave the batched and surfaces keys out of the line")
+    ap.add_argument("--no-extra", action="store_true", help="leave the batched, surfaces and rigid keys out of the line")
     ap.add_argument("--chunk", type=int, default=100, help="steps per captured HIP graph")
     ap.add_argument("--weights", default=os.path.join(ROOT, "feriphys_amd", "weights", "cloth_surrogate.npy"),
                     help="surrogate network for the cloth workload's surrogate leg ('' to skip it)")
@@ -269,6 +270,7 @@ def main():
     if not args.no_extra:
         extra["batched"] = batched_plain(args, np, fa, wl)
         extra["surfaces"] = surfaces_plain(args, np, fa, wl)
+        extra["rigid"], _, _, _ = rigid_plain(args, np, fa)
 
     cpu = None
     if args.cpu_steps > 0:
@@ -683,10 +685,14 @@ def rigid_scene(np, n, seed=0):
     return pos.astype(np.float32), mass.astype(np.float32), imp.astype(np.float32), at.astype(np.float32), floor
 
 
-def bench_rigid(args, np, fa, wl):
-    """1,048,576 unit cubes (the reference's RigidBody) tumbling onto a ground plane, one GPU thread
-    per body.  A "step" is one rigidbody::Simulation::step of every body."""
-    n = 1 << 20
+RIGID_BODIES = 1 << 20
+RIGID_BYTES_PER_BODY_STEP = 2 * 29 * 4  # the 29-float record read and written once per body-step
+
+
+def rigid_plain(args, np, fa):
+    """The 1,048,576-cube rigid step timed over args.steps steps after args.warmup untimed ones (host
+    clock for the rate, HIP events for the kernel); returns (rates, scene pieces, records, dt)."""
+    n = RIGID_BODIES
     pos, mass, imp, at, floor = rigid_scene(np, n)
     bodies = [fa.RigidBody(pos[i], float(mass[i])) for i in range(n)]
     sim = fa.RigidSimulation(bodies, [fa.CollidableMesh(*floor)])
@@ -698,17 +704,31 @@ def bench_rigid(args, np, fa, wl):
     sim.config = fa.RigidConfig(fa.Integration.Rk4, dt, 0.0, (0.0, -9.81, 0.0), (0.0, 0.0, 0.0))
     if args.warmup:
         sim.step(args.warmup)
-        sim.set_states(rec)
+        sim.set_states(rec)  # returns once the records are on the device
     t0 = time.perf_counter()
     kernel_ms = sim.time_steps(args.steps)
     wall = time.perf_counter() - t0
     out = sim.states()
-    if not np.isfinite(out).all():
+    kernel_avg_s = kernel_ms * 1e-3 / args.steps
+    bytes_step = RIGID_BYTES_PER_BODY_STEP * n
+    achieved = bytes_step / kernel_avg_s / 1e9
+    rates = {"workload": f"rigidbody::Simulation::step of {n} unit cubes (random impulses, restitution 0) onto a "
+                         f"ground plane, RK4 dt=1/600 s", "bodies": n, "steps": args.steps,
+             "sim_s_per_s": args.steps * dt / wall, "ms_per_step": wall * 1e3 / args.steps,
+             "body_steps_per_s": args.steps * n / wall, "kernel_avg_us": kernel_avg_s * 1e6,
+             "bytes_per_launch": bytes_step, "achieved_gbs": achieved, "hbm_frac": achieved / HBM_PEAK_GBS,
+             "finite": bool(np.isfinite(out).all())}
+    return rates, (pos, mass, floor), rec, dt
+
+
+def bench_rigid(args, np, fa, wl):
+    """1,048,576 unit cubes (the reference's RigidBody) tumbling onto a ground plane, one GPU thread
+    per body.  A "step" is one rigidbody::Simulation::step of every body."""
+    n = RIGID_BODIES
+    rates, (pos, mass, floor), rec, dt = rigid_plain(args, np, fa)
+    if not rates["finite"]:
         print("non-finite state after the timed steps", file=sys.stderr)
         sys.exit(3)
-    kernel_avg_s = kernel_ms * 1e-3 / args.steps
-    bytes_step = 2 * 29 * 4 * n  # the 29-float record read and written once per body-step
-    achieved = bytes_step / kernel_avg_s / 1e9
     cpu = None
     if args.cpu_steps > 0:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -728,15 +748,14 @@ def bench_rigid(args, np, fa, wl):
                "body_steps_per_s": bsps}
     line = {
         "metric": "sim-s/s, 1M rigid cubes; %HBM peak",
-        "value": args.steps * dt / wall, "unit": "sim-s/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "value": rates["sim_s_per_s"], "unit": "sim-s/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": rates["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"rigidbody::Simulation::step of {n} unit cubes (random impulses, restitution 0) "
-                               f"onto a ground plane, RK4 dt=1/600 s", "bodies": n},
-        "body_steps_per_s": args.steps * n / wall,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel_avg_us": kernel_avg_s * 1e6,
-                     "bytes_per_launch": bytes_step},
+        "config": {"workload": rates["workload"], "bodies": n},
+        "body_steps_per_s": rates["body_steps_per_s"],
+        "roofline": {"bound": "hbm", "achieved": rates["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rates["hbm_frac"], "traffic": None, "kernel_avg_us": rates["kernel_avg_us"],
+                     "bytes_per_launch": rates["bytes_per_launch"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
