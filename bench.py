@@ -170,7 +170,7 @@ def main():
                     help="auto: time both forms and headline the faster")
     ap.add_argument("--kernel", choices=["auto", "grid", "grid_simple", "csr"], default="auto")
     ap.add_argument("--tile", choices=["default", "15x15", "7x31", "16x16"], default="default")
-    ap.add_argument("--cpu-steps", type=int, default=3, help="oracle steps for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=5, help="oracle steps for the CPU baseline (0: skip; 5 on the 1M cloth take about 10 s of one core)")
     ap.add_argument("--workload", choices=["cloth", "batched", "surfaces", "rigid"], default="cloth",
                     help="cloth: one 1024x1024 cloth + sphere (default); batched: 4096 cloths of uneven sizes; "
                          "surfaces: 64 perturbed icospheres (2.1M vertices) on a ground plane; rigid: 1M unit "
