@@ -42,8 +42,9 @@ COMMON = ["step_device.hpp", "vec.hpp", "exact_ops.hpp", "fdlibm_atan2f.hpp", "k
 # the source files each measured kernel is built from; a counter reading below is reported only
 # while these files still hash to the value it was measured on
 KERNEL_SOURCES = {
-    # sim.cpp deals the tiles to the XCDs, which sets how often halo rows are fetched twice
-    "grid_tiled_kernel<ExactMath,16,16>": ["grid_tiled.hip", "sim.cpp"] + COMMON,
+    # sim.cpp deals the tiles to the XCDs, which sets how often halo rows are fetched twice;
+    # obstacles.cpp builds the broad-phase grids the collision search walks
+    "grid_tiled_kernel<ExactMath,16,16>": ["grid_tiled.hip", "obstacles.cpp", "sim.cpp"] + COMMON,
     "grid_fast_kernel<15,15>": ["grid_fast.hip", "grid_fast_core.hpp", "grid_fast.hpp", "obstacles.cpp", "sim.cpp"] + COMMON,
     "patch_kernel<ExactMath>": ["kernels.hip"] + COMMON,
 }
@@ -53,12 +54,12 @@ KERNEL_SOURCES = {
 # GRBM_GUI_ACTIVE summed over the 8 XCDs (the commands are in the profiles named)
 MEASURED = {
     ("grid_tiled_kernel<ExactMath,16,16>", "cloth1024"): {
-        "src_sha": "6f4175c38be33a44", "fetch_kib": 15936, "write_kib": 24576,
-        "valu_insts": 3.5666e7, "waves": 16384, "grbm_gui_active": 1.1268e6,
+        "src_sha": "a097b63c1cc516aa", "fetch_kib": 15935, "write_kib": 24576,
+        "valu_insts": 3.5163e7, "waves": 16384, "grbm_gui_active": 1.1136e6,
         "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
     ("grid_fast_kernel<15,15>", "cloth1024"): {
-        "src_sha": "fa5ca408168d95f6", "fetch_kib": 16278, "write_kib": 24734,
-        "valu_insts": 1.3702e7, "waves": 19044, "grbm_gui_active": 6.1175e5,
+        "src_sha": "fb0860ba5eb7db3b", "fetch_kib": 16218, "write_kib": 24734,
+        "valu_insts": 1.1926e7, "waves": 19044, "grbm_gui_active": 5.3738e5,
         "profiles": ["profiles/grid_fast_cloth1024.md"]},
     ("patch_kernel<ExactMath>", "icospheres64"): {
         "src_sha": "581cd32381980017", "fetch_kib": 216760, "write_kib": 50327,

@@ -9,11 +9,16 @@ namespace fph {
 namespace {
 
 // Obstacle faces and their broad-phase grids (see ObsView).  Faces whose dropped normal component
-// is at least kTau are binned in 3D; those between kThin and kTau in a 2D grid of their dropped
-// axis; the rest are tested for every vertex.  Boxes are grown by kMargin, far above the rounding
-// of the crossing point and of the containment test at the coordinates these scenes use.
-constexpr float kMargin = 1.0e-2f;
-constexpr float kTau = 0.05f;
+// c is at least kTau are binned in 3D; those between kThin and kTau in a 2D grid of their dropped
+// axis (the containment test ignores that coordinate, and a plane nearly parallel to it accepts
+// crossings far along it); the rest are tested for every vertex.  Face and segment boxes are grown
+// by the margin: a crossing point computed with an error of e from the plane is accepted at most
+// e / c from the face along the plane, and e is a few ulps of the coordinates (at most ~5e-7 of
+// the largest obstacle coordinate M), so two margins must exceed 5e-7 M / kTau = 2.5e-5 M; the
+// margin is max(kMinMargin, kRelMargin M), eight times that.
+constexpr float kMinMargin = 1.0e-2f;
+constexpr float kRelMargin = 1.0e-4f;
+constexpr float kTau = 0.02f;
 constexpr float kThin = 1.0e-3f;
 
 int build_grid(ObstacleSet& s, int g, const std::vector<fp::ObsFace>& faces, const std::vector<uint32_t>& ids,
@@ -21,9 +26,10 @@ int build_grid(ObstacleSet& s, int g, const std::vector<fp::ObsFace>& faces, con
   fp::GridView& G = s.view.grid[g];
   G = fp::GridView{};
   if (ids.empty()) return FP_OK;
+  const float margin = s.view.margin;
   auto box = [&](const fp::ObsFace& q, int k, float& l, float& h) {
-    l = std::min(q.v0[k], std::min(q.v1[k], q.v2[k])) - kMargin;
-    h = std::max(q.v0[k], std::max(q.v1[k], q.v2[k])) + kMargin;
+    l = std::min(q.v0[k], std::min(q.v1[k], q.v2[k])) - margin;
+    h = std::max(q.v0[k], std::max(q.v1[k], q.v2[k])) + margin;
   };
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   double ext_sum = 0.0;
@@ -110,7 +116,11 @@ int upload_obstacles(ObstacleSet& s, const std::vector<fp::HostObstacle>& obstac
   O = fp::ObsView{};
   O.faces = s.faces.p;
   O.n_faces = (uint32_t)faces.size();
-  O.margin = kMargin;
+  float coord = 0.0f;  // the largest obstacle coordinate, for the margin
+  for (const auto& f : faces)
+    for (int k = 0; k < 3; ++k)
+      coord = std::max(coord, std::max(std::fabs(f.v0[k]), std::max(std::fabs(f.v1[k]), std::fabs(f.v2[k]))));
+  O.margin = std::max(kMinMargin, kRelMargin * (std::isfinite(coord) ? coord : 0.0f));
   for (int k = 0; k < 3; ++k) {  // no obstacles: an empty box
     s.hit_box.lo[k] = INFINITY;
     s.hit_box.hi[k] = -INFINITY;
@@ -153,8 +163,8 @@ int upload_obstacles(ObstacleSet& s, const std::vector<fp::HostObstacle>& obstac
         const fp::ObsFace& q = faces[f];
         for (int k = 0; k < 3; ++k) {
           const bool open = g > 0 && k == g - 1;
-          const float l = open ? -INFINITY : std::min(q.v0[k], std::min(q.v1[k], q.v2[k])) - 2.0f * kMargin;
-          const float h = open ? INFINITY : std::max(q.v0[k], std::max(q.v1[k], q.v2[k])) + 2.0f * kMargin;
+          const float l = open ? -INFINITY : std::min(q.v0[k], std::min(q.v1[k], q.v2[k])) - 2.0f * O.margin;
+          const float h = open ? INFINITY : std::max(q.v0[k], std::max(q.v1[k], q.v2[k])) + 2.0f * O.margin;
           s.hit_box.lo[k] = std::min(s.hit_box.lo[k], l);
           s.hit_box.hi[k] = std::max(s.hit_box.hi[k], h);
         }

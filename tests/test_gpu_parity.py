@@ -189,10 +189,10 @@ def test_pins_bit_identical():
     assert (bits(after[pins, :7]) == bits(before[pins, :7])).all()
 
 
-def _sphere_scene(n=24):
+def _sphere_scene(n=24, offset=0.0, rotation=wl.SPHERE_ROTATION):
     sc = wl.config0(rows=n, cols=n)
     r = 0.3 * (n - 1) * sc.cloths[0].spacing
-    sc.obstacles = [wl.sphere_obstacle(r, (0.0, -(0.05 + r), 0.0))]
+    sc.obstacles = [wl.sphere_obstacle(r, (offset, -(0.05 + r), offset), rotation=rotation)]
     sc.config.coefficient_of_restitution = 0.3
     sc.config.coefficient_of_friction = 0.2
     return sc
@@ -239,13 +239,18 @@ def test_cloth_on_sphere_collision_trajectory(kernel, math):
     assert e.max() <= 2e-2
 
 
-def test_ccd_first_hit_matches_oracle():
+@pytest.mark.parametrize("offset,rotation", [
+    (0.0, wl.SPHERE_ROTATION),      # every face in the 3D grid
+    (2000.0, wl.SPHERE_ROTATION),   # the same ~2 km from the origin: the margin grows with the coordinates
+    (0.0, (0.005, 0.012, 0.003)),   # faces in the 3D grid, in 2D grids and tested for every vertex
+])
+def test_ccd_first_hit_matches_oracle(offset, rotation):
     """SpringyMesh::update_points against the oracle for random segments around the icosphere
     collider (grid broad phase on the GPU, brute-force first-hit scan in the oracle): identical
     hit choice and bit-identical response."""
     rng = np.random.default_rng(7)
     n = 64
-    sc = _sphere_scene(n)
+    sc = _sphere_scene(n, offset, rotation)
     g = gpu_scene(sc)
     o = oracle_scene(sc)
     sphere_p, sphere_i = sc.obstacles[0]
