@@ -40,11 +40,12 @@ struct ObsFace {
 
 // Broad phase over the obstacle faces: uniform grids of face ids, each cell listing (in ascending
 // id order) every face whose bounding box, grown by `margin`, touches the cell.
-//   grid 0   : 3D grid of faces whose dropped normal component is at least 0.05.  A hit needs the
+//   grid 0   : 3D grid of faces whose dropped normal component is at least 0.02.  A hit needs the
 //              crossing point in the triangle, and the point lies on the segment, so only faces
-//              whose grown box meets the segment's grown box can be hit.
+//              whose grown box meets the segment's grown box can be hit (the margin covers the
+//              crossing point's rounding divided by the dropped component).
 //   grid 1+a : 2D grid (axis a collapsed) of faces that drop axis a with a component in
-//              [1e-3, 0.05).  Their flattened containment test is thin but still confines the
+//              [1e-3, 0.02).  Their flattened containment test is thin but still confines the
 //              crossing point to the triangle's box in the two kept axes; along the dropped axis
 //              the plane may be met anywhere, so that axis is not binned.
 //   always   : faces with a smaller (or non-finite) dropped component, tested for every vertex.
