@@ -54,12 +54,12 @@ KERNEL_SOURCES = {
 # GRBM_GUI_ACTIVE summed over the 8 XCDs (the commands are in the profiles named)
 MEASURED = {
     ("grid_tiled_kernel<ExactMath,16,16>", "cloth1024"): {
-        "src_sha": "64901231c12558d7", "fetch_kib": 15935, "write_kib": 24576,
-        "valu_insts": 3.5163e7, "waves": 16384, "grbm_gui_active": 1.1136e6,
+        "src_sha": "1b87b40915f28406", "fetch_kib": 15936, "write_kib": 24576,
+        "valu_insts": 3.5163e7, "waves": 16384, "grbm_gui_active": 1.1038e6,
         "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
     ("grid_fast_kernel<15,15>", "cloth1024"): {
-        "src_sha": "c86d32f626e7a908", "fetch_kib": 16218, "write_kib": 24734,
-        "valu_insts": 1.1926e7, "waves": 19044, "grbm_gui_active": 5.3738e5,
+        "src_sha": "cc16a49e6b975c9a", "fetch_kib": 16218, "write_kib": 24734,
+        "valu_insts": 1.1926e7, "waves": 19044, "grbm_gui_active": 5.3851e5,
         "profiles": ["profiles/grid_fast_cloth1024.md"]},
     ("patch_kernel<ExactMath>", "icospheres64"): {
         "src_sha": "9df672e620fde631", "fetch_kib": 216760, "write_kib": 50327,

@@ -21,6 +21,49 @@ constexpr float kRelMargin = 1.0e-4f;
 constexpr float kTau = 0.02f;
 constexpr float kThin = 1.0e-3f;
 
+// Whether triangle (a, b, c) meets the box of centre m and half-sizes h: no separating axis among
+// the box's three axes, the triangle's normal and the nine cross products of its edges with the box
+// axes (the separating-axis theorem for a triangle and a box), in double precision.
+bool triangle_meets_box(const double a[3], const double b[3], const double c[3], const double m[3], const double h[3]) {
+  double v[3][3];
+  for (int k = 0; k < 3; ++k) {
+    v[0][k] = a[k] - m[k];
+    v[1][k] = b[k] - m[k];
+    v[2][k] = c[k] - m[k];
+  }
+  auto separated = [&](const double ax[3]) {  // projections of the triangle and of the box on ax
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = 0; i < 3; ++i) {
+      const double p = v[i][0] * ax[0] + v[i][1] * ax[1] + v[i][2] * ax[2];
+      lo = std::min(lo, p);
+      hi = std::max(hi, p);
+    }
+    const double r = h[0] * std::fabs(ax[0]) + h[1] * std::fabs(ax[1]) + h[2] * std::fabs(ax[2]);
+    return lo > r || hi < -r;
+  };
+  double e[3][3];
+  for (int k = 0; k < 3; ++k) {
+    e[0][k] = v[1][k] - v[0][k];
+    e[1][k] = v[2][k] - v[1][k];
+    e[2][k] = v[0][k] - v[2][k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    const double ax[3] = {k == 0 ? 1.0 : 0.0, k == 1 ? 1.0 : 0.0, k == 2 ? 1.0 : 0.0};
+    if (separated(ax)) return false;
+  }
+  const double n[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                       e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+  if (separated(n)) return false;
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) {
+      // e_i x unit_k
+      const double ax[3] = {k == 0 ? 0.0 : (k == 1 ? -e[i][2] : e[i][1]), k == 1 ? 0.0 : (k == 0 ? e[i][2] : -e[i][0]),
+                            k == 2 ? 0.0 : (k == 0 ? -e[i][1] : e[i][0])};
+      if (separated(ax)) return false;
+    }
+  return true;
+}
+
 int build_grid(ObstacleSet& s, int g, const std::vector<fp::ObsFace>& faces, const std::vector<uint32_t>& ids,
                int collapse) {
   fp::GridView& G = s.view.grid[g];
@@ -71,6 +114,23 @@ int build_grid(ObstacleSet& s, int g, const std::vector<fp::ObsFace>& faces, con
     c0 = fl < 0.0f ? 0 : std::min((int)fl, G.dims[k] - 1);
     c1 = fh >= (float)G.dims[k] ? G.dims[k] - 1 : std::max(0, (int)fh);
   };
+  // In the 3D grid a face goes only to the cells its triangle meets once each cell is grown by the
+  // margin (and by a millionth of a cell against the rounding of the kernel's f32 cell mapping): a
+  // crossing point accepted within the margin of the triangle lies in such a cell, which the
+  // segment's grown box then reaches.  A 2D grid keeps the box test (its collapsed axis is open).
+  auto meets = [&](const fp::ObsFace& q, int x, int y, int z) {
+    if (collapse >= 0) return true;
+    const int idx[3] = {x, y, z};
+    double m[3], h[3];
+    for (int k = 0; k < 3; ++k) {
+      const double w = 1.0 / (double)G.inv_cell[k];
+      m[k] = (double)G.lo[k] + ((double)idx[k] + 0.5) * w;
+      h[k] = 0.5 * w * (1.0 + 1e-6) + (double)margin;
+    }
+    const double a[3] = {q.v0[0], q.v0[1], q.v0[2]}, b[3] = {q.v1[0], q.v1[1], q.v1[2]},
+                 c[3] = {q.v2[0], q.v2[1], q.v2[2]};
+    return triangle_meets_box(a, b, c, m, h);
+  };
   std::vector<uint32_t> start(n_cells + 1, 0);
   auto visit = [&](auto&& fn) {
     for (uint32_t f : ids) {  // ascending face ids -> ascending cell lists
@@ -78,7 +138,8 @@ int build_grid(ObstacleSet& s, int g, const std::vector<fp::ObsFace>& faces, con
       for (int k = 0; k < 3; ++k) range(faces[f], k, r[k][0], r[k][1]);
       for (int z = r[2][0]; z <= r[2][1]; ++z)
         for (int y = r[1][0]; y <= r[1][1]; ++y)
-          for (int x = r[0][0]; x <= r[0][1]; ++x) fn(((uint64_t)z * G.dims[1] + y) * G.dims[0] + x, f);
+          for (int x = r[0][0]; x <= r[0][1]; ++x)
+            if (meets(faces[f], x, y, z)) fn(((uint64_t)z * G.dims[1] + y) * G.dims[0] + x, f);
     }
   };
   visit([&](uint64_t c, uint32_t) { start[c + 1]++; });
