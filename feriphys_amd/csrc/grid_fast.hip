@@ -8,8 +8,9 @@ namespace fp {
 
 namespace {
 
-// Seven waves per SIMD: at most 72 VGPRs per lane, and with 15x15 tiles 22.7 KiB of LDS per
-// workgroup, seven workgroups per CU (7x31 tiles take 23.0 KiB: six).
+// At most 72 VGPRs per lane (room for seven waves per SIMD); the LDS, 23,232 B per workgroup with
+// 15x15 tiles, allocated in 512-byte granules, holds six workgroups per CU.  (Entries of 12 bytes
+// in the strut table bring it to 22,960 B and seven workgroups per CU: measured no faster.)
 template <int TR, int TC, int MODE>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7, 8)))
 grid_fast_kernel(StepArgs A, const ClothView* __restrict__ cloths, const TileRange* __restrict__ tiles, HitBox hb) {

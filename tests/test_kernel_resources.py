@@ -1,8 +1,10 @@
 """Register and LDS budgets of the step kernels, read from the gfx950 compiler's own kernel metadata
 (device-only assembly of the library's sources, built here with the library's flags).
 
-The fast grid kernel runs seven workgroups per CU on 15x15 tiles: that needs at most 72 VGPRs per
-lane (seven waves per SIMD) and at most 160 KiB / 7 of LDS per workgroup.  No step kernel may spill
+The fast grid kernel runs six workgroups per CU on 15x15 tiles: it is built for at most 72 VGPRs
+per lane (room for seven waves per SIMD), and six LDS allocations fit in 160 KiB, each rounded up
+to the 512-byte granule the hardware allocates in (the runtime's occupancy query rounds finer and
+reports seven for 23,232 B, where six ran concurrently).  No step kernel may spill
 vector registers to scratch memory: a spill costs HBM traffic per launch (each wave's scratch lines
 are written back at the end of the kernel).  Scalar registers may spill: they go to lanes of a
 vector register, not to memory (in the fast kernel only on the bounds-checked tiles' obstacle
@@ -17,6 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "feriphys_amd", "csrc")
 HIPCC = "/opt/rocm/bin/hipcc"
 LDS_PER_CU = 160 * 1024
+LDS_GRANULE = 512
 VGPRS_PER_SIMD_LANE = 512
 
 pytestmark = pytest.mark.skipif(not os.path.exists(HIPCC), reason="needs hipcc")
@@ -63,13 +66,14 @@ def test_fast_kernel_never_spills(fast_kernels):
         assert m["private_segment_fixed_size"] == 0, name
 
 
-def test_fast_kernel_15x15_fits_seven_workgroups_per_cu(fast_kernels):
+def test_fast_kernel_15x15_fits_six_workgroups_per_cu(fast_kernels):
     k1515 = {n: m for n, m in fast_kernels.items() if "Li15ELi15E" in n}
     assert len(k1515) == 2
     for name, m in k1515.items():
         waves = VGPRS_PER_SIMD_LANE // (-(-m["vgpr_count"] // 8) * 8)
         assert waves >= 7, (name, m["vgpr_count"])
-        assert 7 * m["group_segment_fixed_size"] <= LDS_PER_CU, (name, m["group_segment_fixed_size"])
+        lds = -(-m["group_segment_fixed_size"] // LDS_GRANULE) * LDS_GRANULE
+        assert 6 * lds <= LDS_PER_CU, (name, m["group_segment_fixed_size"])
 
 
 def test_exact_tiled_kernel_never_spills(tiled_kernels):
