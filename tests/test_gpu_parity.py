@@ -214,14 +214,20 @@ def test_cloth_on_sphere_bit_exact_without_face_terms(kernel):
     assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
 
 
-@pytest.mark.parametrize("kernel,math", [(fa.FP_KERNEL_GRID, fa.FP_MATH_EXACT), (fa.FP_KERNEL_CSR, fa.FP_MATH_EXACT),
-                                         (fa.FP_KERNEL_GRID, fa.FP_MATH_FAST)])
-def test_cloth_on_sphere_collision_trajectory(kernel, math):
+@pytest.mark.parametrize("kernel,math,rotation", [
+    (fa.FP_KERNEL_GRID, fa.FP_MATH_EXACT, wl.SPHERE_ROTATION),
+    (fa.FP_KERNEL_CSR, fa.FP_MATH_EXACT, wl.SPHERE_ROTATION),
+    (fa.FP_KERNEL_GRID, fa.FP_MATH_FAST, wl.SPHERE_ROTATION),
+    # faces in the 3D grid, in 2D grids and tested for every vertex: the fast kernel's obstacle box
+    # is then open, and every vertex searches
+    (fa.FP_KERNEL_GRID, fa.FP_MATH_FAST, (0.005, 0.012, 0.003)),
+])
+def test_cloth_on_sphere_collision_trajectory(kernel, math, rotation):
     """The same with wind, drag and lift.  EXACT: bit-identical to the oracle.  FAST: face forces
     differ by ulps, and a vertex that then meets the sphere one step earlier or later diverges by
     up to the distance it travels in a step, so the bar is on the distribution: 99% of vertices
     within 5e-3 m of the oracle and every vertex within 2e-2 m (one step of travel is ~1e-2 m)."""
-    sc = _sphere_scene()
+    sc = _sphere_scene(rotation=rotation)
     sc.steps = 240
     g = gpu_scene(sc, kernel, math)
     g.step(sc.steps)
