@@ -58,8 +58,8 @@ MEASURED = {
         "valu_insts": 3.5163e7, "waves": 16384, "grbm_gui_active": 1.1038e6,
         "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
     ("grid_fast_kernel<15,15>", "cloth1024"): {
-        "src_sha": "465b42acfdf3b67c", "fetch_kib": 16218, "write_kib": 24734,
-        "valu_insts": 1.1926e7, "waves": 19044, "grbm_gui_active": 5.3851e5,
+        "src_sha": "da0c64b10b2b6420", "fetch_kib": 16221, "write_kib": 24734,
+        "valu_insts": 1.1775e7, "waves": 19044, "grbm_gui_active": 5.2552e5,
         "profiles": ["profiles/grid_fast_cloth1024.md"]},
     ("patch_kernel<ExactMath>", "icospheres64"): {
         "src_sha": "9df672e620fde631", "fetch_kib": 216760, "write_kib": 50327,

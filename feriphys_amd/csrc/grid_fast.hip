@@ -31,14 +31,21 @@ grid_fast_kernel(StepArgs A, const ClothView* __restrict__ cloths, const TileRan
       if (hit >= 0) collide_response(A.obs.faces[hit], P, P.mass, p, v, F, xn, vn);
     }
   };
+  // Phase 0 runs at a raised issue priority: a workgroup that has just started gets its loads out
+  // and its strut table computed ahead of the phase-1 arithmetic of the workgroups already resident
+  // on the CU (measured 24.1 -> 23.8 us per step on a 1024 x 1024 cloth).
   if (gfast::interior_tile<TR, TC>(G, r0, c0)) {
+    __builtin_amdgcn_s_setprio(2);
     gfast::phase0<TR, TC, true>(L, A, G, r0, c0, tid, (int)kBlock);
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     const f3 toA = gfast::phase1<TR, TC, true>(L, P, G, r0, c0, tid);
     __syncthreads();
     gfast::phase2<TR, TC, MODE, true>(L, A, P, G, r0, c0, tid, toA, collide);
   } else {
+    __builtin_amdgcn_s_setprio(2);
     gfast::phase0<TR, TC, false>(L, A, G, r0, c0, tid, (int)kBlock);
+    __builtin_amdgcn_s_setprio(0);
     __syncthreads();
     const f3 toA = gfast::phase1<TR, TC, false>(L, P, G, r0, c0, tid);
     __syncthreads();

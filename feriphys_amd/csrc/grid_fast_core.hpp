@@ -174,33 +174,62 @@ template <int TR, int TC, bool INTERIOR>
 FP_HD void phase0(Lds<TR, TC>& L, const StepArgs& A, const ClothView& G, int r0, int c0, int tid, int nthreads) {
   using S = Lds<TR, TC>;
   const int R = (int)G.rows, C = (int)G.cols;
+  // The staged region has more entries than the workgroup has threads (361 or 385 for 256).  Every
+  // round's loads are issued, unconditionally, before the first LDS store, so that no thread waits
+  // for two HBM round trips in a row: a lane past the region's end loads its last entry again, and
+  // on a tile at the grid's edge a lane outside the grid the nearest grid vertex; only the stores
+  // are conditional.  (The rounds assume nthreads == kBlock, the kernel's only block size.)
+  constexpr int NS = S::SH * S::SW, RND = (NS + (int)kBlock - 1) / (int)kBlock;
+  FP_LDS_CHECK(nthreads == (int)kBlock);
+  float4 q[RND];
+  float2 w[RND];
   if (INTERIOR) {
     // the staged region's first vertex (r0-2, c0-2) lies in the grid: every array is addressed
     // from its (workgroup-uniform) pointer to that vertex plus a 32-bit byte offset per lane
     const uint32_t g0 = G.base + (uint32_t)((r0 - 2) * C + (c0 - 2));
     const char *bx = (const char*)(A.in.x + g0), *by = (const char*)(A.in.y + g0), *bz = (const char*)(A.in.z + g0);
     const char *bu = (const char*)(A.in.vx + g0), *bv = (const char*)(A.in.vy + g0), *bw = (const char*)(A.in.vz + g0);
-    for (int e = tid; e < S::SH * S::SW; e += nthreads) {
-      const int lr = e / S::SW, lc = e - lr * S::SW;
+#pragma unroll
+    for (int k = 0; k < RND; ++k) {
+      const int e = tid + k * nthreads < NS ? tid + k * nthreads : NS - 1, lr = e / S::SW, lc = e - lr * S::SW;
       const uint32_t ob = (uint32_t)(lr * C + lc) * 4u;
-      FP_LDS_CHECK(e >= 0 && e < S::SH * S::SW);
-      L.pv[e] = float4{*(const float*)(bx + ob), *(const float*)(by + ob), *(const float*)(bz + ob),
-                       *(const float*)(bu + ob)};
-      L.vw[e] = float2{*(const float*)(bv + ob), *(const float*)(bw + ob)};
+      q[k] = float4{*(const float*)(bx + ob), *(const float*)(by + ob), *(const float*)(bz + ob),
+                    *(const float*)(bu + ob)};
+      w[k] = float2{*(const float*)(bv + ob), *(const float*)(bw + ob)};
+    }
+#pragma unroll
+    for (int k = 0; k < RND; ++k) {
+      const int e = tid + k * nthreads;
+      if (e < NS) {
+        FP_LDS_CHECK(e >= 0 && e < NS);
+        L.pv[e] = q[k];
+        L.vw[e] = w[k];
+      }
     }
   } else {
-    for (int e = tid; e < S::SH * S::SW; e += nthreads) {
+#pragma unroll
+    for (int k = 0; k < RND; ++k) {
+      const int e = tid + k * nthreads < NS ? tid + k * nthreads : NS - 1, lr = e / S::SW, lc = e - lr * S::SW;
+      const int rr = r0 - 2 + lr, cc = c0 - 2 + lc;
+      const int r = rr < 0 ? 0 : rr >= R ? R - 1 : rr, c = cc < 0 ? 0 : cc >= C ? C - 1 : cc;
+      const uint32_t g = G.base + (uint32_t)(r * C + c);
+      q[k] = float4{A.in.x[g], A.in.y[g], A.in.z[g], A.in.vx[g]};
+      w[k] = float2{A.in.vy[g], A.in.vz[g]};
+    }
+#pragma unroll
+    for (int k = 0; k < RND; ++k) {
+      const int e = tid + k * nthreads;
       const int lr = e / S::SW, lc = e - lr * S::SW;
       const int r = r0 - 2 + lr, c = c0 - 2 + lc;
-      if (r < 0 || r >= R || c < 0 || c >= C) continue;
-      const uint32_t g = G.base + (uint32_t)(r * C + c);
-      FP_LDS_CHECK(e >= 0 && e < S::SH * S::SW);
-      L.pv[e] = float4{A.in.x[g], A.in.y[g], A.in.z[g], A.in.vx[g]};
-      L.vw[e] = float2{A.in.vy[g], A.in.vz[g]};
+      if (e < NS && r >= 0 && r < R && c >= 0 && c < C) {
+        FP_LDS_CHECK(e >= 0 && e < NS);
+        L.pv[e] = q[k];
+        L.vw[e] = w[k];
+      }
     }
   }
-  // the tables are computed by the upper half of the workgroup (waves 2 and 3 of 4), whose share of
-  // the staging above is one round of loads where the lower half has two
+  // the tables are computed by the upper half of the workgroup (waves 2 and 3 of 4), which store one
+  // round of the staging above where the lower half stores two
   const int t0 = nthreads >= 2 * S::NTAB ? nthreads / 2 : 0;
   for (int e = tid - t0; e >= 0 && e < S::NTAB; e += nthreads) {
     // which table: coordinate axis (columns or rows), first index, reach (1 or 2), nominal constants
