@@ -32,11 +32,29 @@ struct StepParams {
 // Obstacle face for the first-hit scan (collidable_mesh.rs:30-47): corners, the unit normal
 // (computed once on the host with the reference's operations; Face::normal recomputes the same
 // value on every call) and the axis the containment test drops (:133-154).
-struct ObsFace {
+struct alignas(16) ObsFace {
   float v0[3], v1[3], v2[3], n[3];
   int32_t axis;  // 0: drop x, 1: drop y, 2: drop z
   int32_t pad[3];
 };
+static_assert(sizeof(ObsFace) == 64, "one 64-byte record per face");
+
+// Face i's corners and normal as three 16-byte loads issued together (the plane test needs the
+// normal and the first corner, the containment test the rest: one round trip serves both), and
+// its dropped axis recomputed from the normal by the comparisons that set it on the host
+// (collidable_mesh.rs:133-154) rather than loaded.
+__device__ __forceinline__ ObsFace load_face(const ObsFace* __restrict__ faces, uint32_t i) {
+  const float4* q = reinterpret_cast<const float4*>(faces + i);
+  const float4 a = q[0], b = q[1], c = q[2];
+  ObsFace f;
+  f.v0[0] = a.x; f.v0[1] = a.y; f.v0[2] = a.z;
+  f.v1[0] = a.w; f.v1[1] = b.x; f.v1[2] = b.y;
+  f.v2[0] = b.z; f.v2[1] = b.w; f.v2[2] = c.x;
+  f.n[0] = c.y; f.n[1] = c.z; f.n[2] = c.w;
+  f.axis = (c.y >= c.z && c.y >= c.w) ? 0 : ((c.z >= c.y && c.z >= c.w) ? 1 : 2);
+  f.pad[0] = f.pad[1] = f.pad[2] = 0;
+  return f;
+}
 
 // Broad phase over the obstacle faces: uniform grids of face ids, each cell listing (in ascending
 // id order) every face whose bounding box, grown by `margin`, touches the cell.
@@ -225,10 +243,18 @@ __device__ __forceinline__ void grid_hits(const GridView& G, const ObsFace* face
     for (int32_t cy = c0[1]; cy <= c1[1]; ++cy)
       for (int32_t cx = c0[0]; cx <= c1[0]; ++cx) {
         const uint32_t cell = ((uint32_t)cz * (uint32_t)G.dims[1] + (uint32_t)cy) * (uint32_t)G.dims[0] + (uint32_t)cx;
-        for (uint32_t k = G.cell_start[cell], e = G.cell_start[cell + 1]; k < e; ++k) {
-          const uint32_t f = G.cell_faces[k];
-          if (f >= best) break;
-          if (face_hit(faces[f], old_p, new_p, dt_d)) { best = f; break; }
+        // the cell's faces in ascending order, each index loaded while the one before is tested
+        uint32_t k = G.cell_start[cell];
+        const uint32_t e = G.cell_start[cell + 1];
+        uint32_t f = k < e ? G.cell_faces[k] : 0xffffffffu;
+        while (f < best) {
+          ++k;
+          const uint32_t next = k < e ? G.cell_faces[k] : 0xffffffffu;
+          if (face_hit(load_face(faces, f), old_p, new_p, dt_d)) {
+            best = f;
+            break;
+          }
+          f = next;
         }
       }
 }
@@ -240,14 +266,14 @@ __device__ __forceinline__ int32_t first_hit(const ObsView& O, f3 old_p, f3 new_
   if (O.n_faces == 0) return -1;
   if (!O.use_grid) {
     for (uint32_t f = 0; f < O.n_faces; ++f)
-      if (face_hit(O.faces[f], old_p, new_p, dt_d)) return (int32_t)f;
+      if (face_hit(load_face(O.faces, f), old_p, new_p, dt_d)) return (int32_t)f;
     return -1;
   }
   uint32_t best = 0xffffffffu;
   for (uint32_t k = 0; k < O.n_always; ++k) {
     const uint32_t f = O.always[k];
     if (f >= best) break;
-    if (face_hit(O.faces[f], old_p, new_p, dt_d)) { best = f; break; }
+    if (face_hit(load_face(O.faces, f), old_p, new_p, dt_d)) { best = f; break; }
   }
   const float lo[3] = {fminf(old_p.x, new_p.x) - O.margin, fminf(old_p.y, new_p.y) - O.margin,
                        fminf(old_p.z, new_p.z) - O.margin};
