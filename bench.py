@@ -54,15 +54,15 @@ KERNEL_SOURCES = {
 # GRBM_GUI_ACTIVE summed over the 8 XCDs (the commands are in the profiles named)
 MEASURED = {
     ("grid_tiled_kernel<ExactMath,16,16>", "cloth1024"): {
-        "src_sha": "1b87b40915f28406", "fetch_kib": 15936, "write_kib": 24576,
-        "valu_insts": 3.5163e7, "waves": 16384, "grbm_gui_active": 1.1038e6,
+        "src_sha": "4a401cbd77d5d46f", "fetch_kib": 15887, "write_kib": 24576,
+        "valu_insts": 3.5014e7, "waves": 16384, "grbm_gui_active": 1.1058e6,
         "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
     ("grid_fast_kernel<15,15>", "cloth1024"): {
-        "src_sha": "da0c64b10b2b6420", "fetch_kib": 16221, "write_kib": 24734,
-        "valu_insts": 1.1775e7, "waves": 19044, "grbm_gui_active": 5.2552e5,
+        "src_sha": "8c7fae5a56e575e1", "fetch_kib": 16222, "write_kib": 24734,
+        "valu_insts": 1.1775e7, "waves": 19044, "grbm_gui_active": 5.3371e5,
         "profiles": ["profiles/grid_fast_cloth1024.md"]},
     ("patch_kernel<ExactMath>", "icospheres64"): {
-        "src_sha": "9df672e620fde631", "fetch_kib": 216760, "write_kib": 50327,
+        "src_sha": "267a4afff89029a1", "fetch_kib": 216816, "write_kib": 50338,
         "profiles": ["profiles/batched_and_surfaces.md"]},
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
