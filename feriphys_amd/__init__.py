@@ -8,13 +8,13 @@ from ._lib import (FP_EULER, FP_KERNEL_AUTO, FP_KERNEL_CSR, FP_KERNEL_GRID, FP_K
                    FP_PLANE_XY, FP_PLANE_XZ, FP_RK4, FP_STEP_CSR, FP_STEP_GRID_SIMPLE, FP_STEP_PATCH, FP_STEP_TILED,
                    FP_TILE_7x31, FP_TILE_15x15, FP_TILE_16x16, FpError, lib)
 from .springy import (Cloth, CollidableMesh, Config, Integration, Simulation, SpringyMesh, config_default,
-                      selftest_exact_math)
+                      pinned_empty, selftest_exact_math)
 from .rigid import RigidBody, RigidConfig, RigidSimulation, rigid_config_default
 from .surrogate import Surrogate, SurrogateConfig, SurrogateStats
 
 __all__ = [
     "Cloth", "CollidableMesh", "Config", "Integration", "Simulation", "SpringyMesh", "config_default", "FpError",
-    "selftest_exact_math",
+    "selftest_exact_math", "pinned_empty",
     "Surrogate", "SurrogateConfig", "SurrogateStats", "RigidBody", "RigidConfig", "RigidSimulation",
     "rigid_config_default",
     "lib", "FP_EULER", "FP_RK4", "FP_PLANE_XY", "FP_PLANE_XZ", "FP_KERNEL_AUTO", "FP_KERNEL_CSR", "FP_KERNEL_GRID",

@@ -118,6 +118,8 @@ SIGNATURES = {
     "fp_sim_get_points": (C.c_int, [_P, _U64, _FP]),
     "fp_sim_set_points": (C.c_int, [_P, _U64, _FP, _U64]),
     "fp_sim_get_vertices": (C.c_int, [_P, _U64, _FP, _U64P]),
+    "fp_host_alloc": (C.c_int, [_U64, _PP]),
+    "fp_host_free": (C.c_int, [_P]),
     "fp_sim_accumulate_forces": (C.c_int, [_P, _U64]),
     "fp_sim_update_points": (C.c_int, [_P, _U64, _FP, _U64]),
     "fp_sim_clear_forces": (C.c_int, [_P, _U64]),

@@ -1154,6 +1154,19 @@ int fp_sim_get_vertices(fp_sim* s, uint64_t mesh, float* pos, uint64_t* idx) {
   return FP_OK;
 }
 
+int fp_host_alloc(uint64_t bytes, void** out) {
+  if (!out) return fail(FP_EINVAL, "out is NULL");
+  *out = nullptr;
+  if (bytes == 0) return FP_OK;
+  HIP_TRY(hipHostMalloc(out, (size_t)bytes, hipHostMallocDefault));
+  return FP_OK;
+}
+
+int fp_host_free(void* ptr) {
+  if (ptr) HIP_TRY(hipHostFree(ptr));
+  return FP_OK;
+}
+
 int fp_sim_accumulate_forces(fp_sim* s, uint64_t mesh) {
   if (int e = check_sim_mesh(s, mesh)) return e;
   const MeshSlot& m = s->meshes[mesh];

@@ -291,9 +291,14 @@ class Simulation(_Handle):
         check(lib().fp_sim_mesh_counts(self.handle, mesh, *[C.byref(x) for x in v]))
         return tuple(x.value for x in v)
 
-    def get_points(self, mesh: int = 0) -> np.ndarray:
+    def get_points(self, mesh: int = 0, out: np.ndarray = None) -> np.ndarray:
+        """The mesh's 10-float records; `out`, when given (a C-contiguous float32 array of shape
+        (points, 10), e.g. a pinned buffer a render loop reuses), receives them instead of a new array."""
         N = self.mesh_counts(mesh)[0]
-        out = np.zeros((N, 10), np.float32)
+        if out is None:
+            out = np.zeros((N, 10), np.float32)
+        elif out.shape != (N, 10) or out.dtype != np.float32 or not out.flags.c_contiguous:
+            raise ValueError(f"out must be a C-contiguous float32 array of shape ({N}, 10)")
         check(lib().fp_sim_get_points(self.handle, mesh, _fptr(out)))
         return out
 
@@ -423,3 +428,31 @@ def selftest_exact_math() -> dict:
     check(lib().fp_selftest_exact_math(out))
     return {"rcp": int(out[0]), "div_pi": int(out[1]), "div3": int(out[2]), "atan2": int(out[3]),
             "sqrt": int(out[4])}
+
+
+class _PinnedBlock:
+    """Page-locked host memory from fp_host_alloc, exposed as a NumPy array interface and released
+    with fp_host_free when the last array viewing it is collected."""
+
+    def __init__(self, shape, dtype):
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape)) * dt.itemsize
+        self.ptr = C.c_void_p()
+        check(lib().fp_host_alloc(n, C.byref(self.ptr)))
+        self.__array_interface__ = {"shape": tuple(shape), "typestr": dt.str, "data": (self.ptr.value, False),
+                                    "version": 3}
+
+    def __del__(self):
+        if self.ptr is not None and self.ptr.value:
+            lib().fp_host_free(self.ptr)
+            self.ptr = None
+
+
+def pinned_empty(shape, dtype=np.float32) -> np.ndarray:
+    """An uninitialised array in page-locked host memory (fp_host_alloc): get_points, set_points and
+    get_vertices copy to and from it at the bus's full rate, as a render loop's reused staging
+    buffer.  No counterpart in the reference."""
+    shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+    if int(np.prod(shape)) == 0:
+        return np.empty(shape, dtype)
+    return np.asarray(_PinnedBlock(shape, dtype))

@@ -191,6 +191,13 @@ int fp_sim_set_points(fp_sim* sim, uint64_t mesh, const float* records, uint64_t
  * indices of every face.  Either output may be NULL; a renderer that keeps the (static) index buffer
  * passes NULL for it and gets the positions in one 12-byte-per-vertex copy. */
 int fp_sim_get_vertices(fp_sim* sim, uint64_t mesh, float* positions, uint64_t* indices);
+/* No counterpart in the reference, whose records live in ordinary Vec<Point> memory.  Page-locked
+ * host memory for the buffers of the readback and restore calls above, for callers that read the
+ * state back every frame (render loops): a copy to or from it runs at the bus's full rate, where a
+ * pageable buffer is staged through the runtime's own pinned chunks.  `*out` is NULL for bytes == 0;
+ * fp_host_free releases what fp_host_alloc returned (NULL is a no-op). */
+int fp_host_alloc(uint64_t bytes, void** out);
+int fp_host_free(void* ptr);
 /* SpringyMesh::accumulate_forces (springy_mesh.rs:515-525) with the mesh's config: gravity,
  * struts, faces, pinned F = 0.  The forces appear in fp_sim_get_points records until cleared. */
 int fp_sim_accumulate_forces(fp_sim* sim, uint64_t mesh);

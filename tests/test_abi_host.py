@@ -1,5 +1,6 @@
 """CPU tests of the C ABI library: it loads, exports every declared symbol, and its host-side
 constructors (no GPU needed) build the same tables as the oracle, bit for bit."""
+import ctypes as C
 import os
 import re
 import subprocess
@@ -89,6 +90,17 @@ def test_rust_binding_declares_every_entry_point():
         n_c = 0 if c_args in ("", "void") else c_args.count(",") + 1
         n_r = 0 if not args.strip() else args.count(",") + 1
         assert n_c == n_r, name
+
+
+def test_host_alloc_paths_without_a_device():
+    """fp_host_alloc's checks that need no GPU: a NULL out pointer is FP_EINVAL, zero bytes give
+    NULL and FP_OK, and fp_host_free(NULL) is a no-op."""
+    L = _lib.lib()
+    assert L.fp_host_alloc(16, None) == _lib.FP_EINVAL
+    p = C.c_void_p(12345)
+    assert L.fp_host_alloc(0, C.byref(p)) == _lib.FP_OK and not p.value
+    assert L.fp_host_free(None) == _lib.FP_OK
+    assert fa.pinned_empty((0, 10)).shape == (0, 10)
 
 
 def test_config_default():

@@ -1,7 +1,8 @@
 """Degenerate and boundary inputs on the GPU, each against the oracle: a simulation with no meshes,
 a general mesh with vertices that no triangle uses (patches of one vertex), pinned vertices on the
 patch kernel, a single triangle, a mesh of struts without faces, a zero timestep, per-mesh counts of
-non-finite vertices, and a rigid-body simulation with no bodies."""
+non-finite vertices, and a rigid-body simulation with no bodies; and records read into a reused
+buffer, against a fresh read."""
 import numpy as np
 import pytest
 
@@ -129,3 +130,24 @@ def test_zero_timestep_leaves_the_state():
     assert g.step(3) == 0.0
     o.step(3)
     assert (bits(g.get_points(0)) == bits(o.mesh(0).points())).all()
+
+
+def test_get_points_into_a_reused_buffer():
+    """get_points(out=...) fills the caller's buffer (as a render loop reuses its staging buffer, here
+    also one in page-locked memory from fp_host_alloc) with the records a fresh get_points returns,
+    bit for bit, and refuses a buffer of the wrong shape."""
+    announce("get_points into a reused buffer: 24 x 17 cloth, 7 steps")
+    cloth = fa.Cloth(24, 17, 0.05, (0.0, 1.0, 0.0), 0.01, 20.0, 0.1, 15.0, 0.1, 5.0, 0.05, [0, 16])
+    g = fa.Simulation([cloth.mesh], [fa.CollidableMesh(*FLOOR)])
+    g.config = cfg_pair()[0]
+    buf = np.full((24 * 17, 10), np.nan, np.float32)
+    for _ in range(7):
+        g.step(1)
+        got = g.get_points(0, out=buf)
+        assert got is buf
+        assert (bits(buf) == bits(g.get_points(0))).all()
+    pinned = fa.pinned_empty((24 * 17, 10))
+    assert pinned.shape == (24 * 17, 10) and pinned.dtype == np.float32
+    assert (bits(g.get_points(0, out=pinned)) == bits(g.get_points(0))).all()
+    with pytest.raises(ValueError):
+        g.get_points(0, out=np.zeros((24 * 17, 9), np.float32))
