@@ -239,7 +239,7 @@ def main():
         print(json.dumps(headline_line(args, legs, best, None, None, {}, N, dt, scene)), flush=True)
         return
 
-    # the two forms over the surrogate's 600-step horizon: the exact run is the surrogate's judge
+    # the two forms over the surrogate's 600-step horizon: the exact run is what the surrogate is measured against
     horizon_chunk = graph_chunk(SURROGATE_HORIZON, args.chunk)
     ref_exact = run_from(sim, fa, kern, "exact", initial, SURROGATE_HORIZON, horizon_chunk)
     compare = {"steps": SURROGATE_HORIZON}
@@ -358,7 +358,7 @@ def surrogate_leg(args, np, fa, wl, sim, kern, ref_exact, initial, N, dt, legs):
     full 600-step horizon whatever --steps is, timed on the host clock around the whole run (it
     reads each big step's diagnostics back); the network kernel's launches timed with HIP events.
     Error: the final positions against the exact plain run after the same number of dt (the plain
-    step of springy/simulation.rs:27-45 is the judge)."""
+    step of springy/simulation.rs:27-45 is the reference for the error)."""
     if not args.weights or not os.path.exists(args.weights):
         return None
     params = np.load(args.weights, allow_pickle=False)

@@ -3,8 +3,8 @@
  * This is the drop-in boundary for the springy-mesh path of the reference crate
  * (src/simulation/springy/, src/simulation/state.rs, src/simulation/collidable_mesh.rs).  The
  * reference is a Rust binary crate with no FFI; each entry point below replaces one Rust method
- * and cites it (file:line).  A Rust caller binds these with `extern "C"` declarations (see
- * INTEGRATION.md); plain pointers and sizes only.
+ * and cites it (file:line).  A Rust caller binds these with `extern "C"` declarations; plain
+ * pointers and sizes only.
  *
  * Ownership mirrors the Rust API: meshes and obstacles are built on the host, then MOVED into a
  * simulation (fp_sim_new consumes the handles, as Simulation::new takes Vec<SpringyMesh> by value).

@@ -77,7 +77,7 @@ def test_bench_refuses_more_than_one_gpu():
 
 
 def test_rust_binding_declares_every_entry_point():
-    """The appendix of INTEGRATION.md (the extern "C" block a Rust caller adds) declares each
+    """The integration guide's appendix (the extern "C" block a Rust caller adds) declares each
     function of the header once, with the header's parameter count."""
     doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     block = doc[doc.index("## Appendix: every entry point"):]

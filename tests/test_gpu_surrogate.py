@@ -228,7 +228,7 @@ WEIGHTS = wl.__file__.replace("workloads.py", "weights/cloth_surrogate.npy")
 def test_surrogate_meets_its_bar_over_600_steps():
     """The contract, with the fallback on at the library's default thresholds: the 1024 x 1024 cloth
     with its sphere over 600 dt (75 big steps of 8 dt) ends within 1e-3 RMS of the cloth width of
-    the plain exact run (the judge: springy/simulation.rs:27-45 stepped 600 times), and the network
+    the plain exact run (the reference for the error: springy/simulation.rs:27-45 stepped 600 times), and the network
     carried at least one big step (the fallback did not simply redo every step)."""
     sc = wl.config1(steps=600)
     g = wl.build_gpu(sc)

@@ -1,4 +1,4 @@
-"""Pins the surrogate network's CPU oracle (oracle/mlp_oracle.py) before it judges the GPU path:
+"""Pins the surrogate network's CPU oracle (oracle/mlp_oracle.py) before it checks the GPU path:
 the analytic gradient against central finite differences, the piecewise-quadratic activation against
 the exact (erf) GELU it follows, hand-computed features, and the bf16 rounding rule."""
 import numpy as np
