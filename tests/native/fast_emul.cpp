@@ -3,6 +3,7 @@
 // LDS filled with NaN beforehand, so a read of a slot that no phase wrote shows in the results.
 // Test infrastructure only: it lets the tests compare the kernel's tile logic with the oracle
 // without a GPU.  No obstacles (the collision functions are the exact kernels' own).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -83,14 +84,20 @@ int emul_fast(uint32_t rows, uint32_t cols, float spacing, const float* position
   P.friction = cfg[11];
   P.mass = cfg[12];
   P.uniform_mass = 1;
-  std::vector<float> st(state, state + 6 * (size_t)n);
+  // every plane in an allocation of its own, so that an address sanitizer reports any access past
+  // a plane's end (one contiguous buffer would hide such a read in the next plane)
+  std::vector<std::vector<float>> in(6), o(6, std::vector<float>(n)), f(3);
+  for (int k = 0; k < 6; ++k) in[k].assign(state + (size_t)k * n, state + (size_t)(k + 1) * n);
+  for (int k = 0; k < 3; ++k) f[k].assign(forces + (size_t)k * n, forces + (size_t)(k + 1) * n);
+  std::vector<uint32_t> pin_words;
+  if (pins) pin_words.assign(pins, pins + (n + 31) / 32);
   fp::StepArgs A{};
-  A.in = fp::SoA{st.data(), st.data() + n, st.data() + 2 * n, st.data() + 3 * n, st.data() + 4 * n, st.data() + 5 * n};
-  A.out = fp::SoA{out, out + n, out + 2 * n, out + 3 * n, out + 4 * n, out + 5 * n};
-  A.fx = forces;
-  A.fy = forces + n;
-  A.fz = forces + 2 * n;
-  A.pins = pins;
+  A.in = fp::SoA{in[0].data(), in[1].data(), in[2].data(), in[3].data(), in[4].data(), in[5].data()};
+  A.out = fp::SoA{o[0].data(), o[1].data(), o[2].data(), o[3].data(), o[4].data(), o[5].data()};
+  A.fx = f[0].data();
+  A.fy = f[1].data();
+  A.fz = f[2].data();
+  A.pins = pins ? pin_words.data() : nullptr;
   A.force_init = force_init;
   A.obs = fp::ObsView{};
   if (shape == 0) {
@@ -100,6 +107,8 @@ int emul_fast(uint32_t rows, uint32_t cols, float spacing, const float* position
     if (mode == 0) run<7, 31, fp::MODE_STEP>(A, G, P);
     else run<7, 31, fp::MODE_FORCES>(A, G, P);
   }
+  for (int k = 0; k < 6; ++k) std::copy(o[k].begin(), o[k].end(), out + (size_t)k * n);
+  for (int k = 0; k < 3; ++k) std::copy(f[k].begin(), f[k].end(), forces + (size_t)k * n);
   return 0;
 }
 

@@ -165,3 +165,21 @@ def test_emulated_wind_dominated_faces(emul):
     assert np.abs(fr).max() > 1e-3
     fe = run_emul(emul, sc, rec, 1, 0)[:, 7:10]
     assert force_rel_err(fe, fr) <= 1e-5
+
+
+def test_emulated_staging_stays_in_bounds_under_address_sanitizer(tmp_path):
+    """The emulation built with an address sanitizer (host code only) and driven over 100 grids of
+    sides 2-34, both tile shapes and both modes (tests/native/fast_emul_asan.cpp): every state
+    plane in an allocation of its own, so that a staging load past a plane's end (the clamped loads
+    of lanes past the staged region or outside the grid) or a pin-word read past the mask would be
+    reported; none is."""
+    exe = str(tmp_path / "fast_emul_asan")
+    native = os.path.join(ROOT, "tests", "native")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-x", "hip", "--cuda-host-only", "--offload-arch=gfx950", "-std=c++17", "-O1",
+                    "-g", "-ffp-contract=off", "-fno-fast-math",
+                    *"-Xarch_host -fsanitize=address -Xarch_host -fno-omit-frame-pointer".split(),
+                    os.path.join(native, "fast_emul.cpp"), os.path.join(native, "fast_emul_asan.cpp"), "-o", exe],
+                   check=True, stderr=subprocess.DEVNULL)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=1:halt_on_error=1"))
+    assert r.returncode == 0 and r.stdout.strip() == "ok", (r.stdout[-2000:], r.stderr[-4000:])
