@@ -54,12 +54,12 @@ KERNEL_SOURCES = {
 # GRBM_GUI_ACTIVE summed over the 8 XCDs (the commands are in the profiles named)
 MEASURED = {
     ("grid_tiled_kernel<ExactMath,16,16>", "cloth1024"): {
-        "src_sha": "4a401cbd77d5d46f", "fetch_kib": 15887, "write_kib": 24576,
-        "valu_insts": 3.5014e7, "waves": 16384, "grbm_gui_active": 1.1058e6,
+        "src_sha": "326562afa3f32bc6", "fetch_kib": 15887, "write_kib": 24576,
+        "valu_insts": 3.5014e7, "waves": 16384, "grbm_gui_active": 1.0991e6,
         "profiles": ["profiles/grid_tiled_exact_cloth1024.md", "profiles/grid_tiled_cloth1024_pmc.md"]},
     ("grid_fast_kernel<15,15>", "cloth1024"): {
-        "src_sha": "8c7fae5a56e575e1", "fetch_kib": 16222, "write_kib": 24734,
-        "valu_insts": 1.1775e7, "waves": 19044, "grbm_gui_active": 5.3371e5,
+        "src_sha": "765ed741246d1deb", "fetch_kib": 16221, "write_kib": 24734,
+        "valu_insts": 1.1775e7, "waves": 19044, "grbm_gui_active": 5.3408e5,
         "profiles": ["profiles/grid_fast_cloth1024.md"]},
     ("patch_kernel<ExactMath>", "icospheres64"): {
         "src_sha": "267a4afff89029a1", "fetch_kib": 216816, "write_kib": 50338,
