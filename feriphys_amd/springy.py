@@ -297,8 +297,8 @@ class Simulation(_Handle):
         N = self.mesh_counts(mesh)[0]
         if out is None:
             out = np.zeros((N, 10), np.float32)
-        elif out.shape != (N, 10) or out.dtype != np.float32 or not out.flags.c_contiguous:
-            raise ValueError(f"out must be a C-contiguous float32 array of shape ({N}, 10)")
+        elif out.shape != (N, 10) or out.dtype != np.float32 or not out.flags.c_contiguous or not out.flags.writeable:
+            raise ValueError(f"out must be a writeable C-contiguous float32 array of shape ({N}, 10)")
         check(lib().fp_sim_get_points(self.handle, mesh, _fptr(out)))
         return out
 
@@ -443,9 +443,12 @@ class _PinnedBlock:
                                     "version": 3}
 
     def __del__(self):
-        if self.ptr is not None and self.ptr.value:
-            lib().fp_host_free(self.ptr)
-            self.ptr = None
+        ptr, self.ptr = getattr(self, "ptr", None), None
+        if ptr is not None and ptr.value:
+            try:
+                lib().fp_host_free(ptr)
+            except Exception:  # interpreter shutdown: the library may already be unloaded
+                pass
 
 
 def pinned_empty(shape, dtype=np.float32) -> np.ndarray:
